@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Benchmark: SSAM 2D stencil suite on one MI355X.
+
+Workload (one "step"): one sweep of every 2D stencil of the 8192 x 8192 FP32
+configuration -- boxes of radius 1..5 (9/25/49/81/121 points), stars of radius
+3/4/5 (13/17/21 points), and the temporally blocked 5-point and 9-point stars
+fused over T = 8 steps -- on a grid of U[-1,1) values from seed 0 with taps
+U[0,1) normalised to sum 1, inputs resident in HBM.  value = total point-updates /
+time of the step, in Gpoints/s.
+
+Also measured in the same run (outside the timed steps): per-kernel times with
+HIP events on the launch stream, the stream-copy bandwidth, the same suite in FP64
+and in BF16 storage (with max / RMS error of BF16 against the GPU FP64 path on
+identical inputs), and the CPU baseline (the oracle on a bounded slab, all host
+threads).
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+N = 8192
+
+# (name, shape, radius, fused steps)
+SUITE = [
+    ("box9", "box", 1, 1), ("box25", "box", 2, 1), ("box49", "box", 3, 1),
+    ("box81", "box", 4, 1), ("box121", "box", 5, 1),
+    ("star13", "star", 3, 1), ("star17", "star", 4, 1), ("star21", "star", 5, 1),
+    ("tmp5_T8", "star", 1, 8), ("tmp9_T8", "star", 2, 8),
+]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip FP64/BF16 side measurements")
+    ap.add_argument("--only", default="", help="comma list of suite kernel names (profiling)")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the SSAM stencil path runs on a single MI355X here "
+              "(multi-GPU slab decomposition is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from ssam_amd import _lib as L
+    from ssam_amd import api
+    from ssam_amd import workloads as wl
+
+    L.lib()  # fail loudly if libssam.so is missing
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    suite = [s for s in SUITE if not a.only or s[0] in a.only.split(",")]
+
+    def taps(shape, r):
+        return wl.normalised_taps(shape, r, seed=1000 + 10 * r + (shape == "star"))
+
+    def run_suite(src, dst, events=None):
+        for i, (name, shape, r, t) in enumerate(suite):
+            if events is not None:
+                events[i][0].record(stream)
+            api.stencil2d(src, dst, taps(shape, r), shape=L.STAR if shape == "star" else L.BOX,
+                          steps=t, stream=stream)
+            if events is not None:
+                events[i][1].record(stream)
+
+    # ---------------------------------------------------------------- inputs in HBM
+    src = torch.empty((N, N), dtype=torch.float32, device=dev)
+    api.fill_uniform(src, 0, -1.0, 1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+
+    for _ in range(a.warmup):
+        run_suite(src, dst)
+    torch.cuda.synchronize()
+
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in suite] for _ in range(a.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(a.steps):
+        run_suite(src, dst, ev[k])
+    t_stop.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_stop)
+    ms_per_step = total_ms / a.steps
+
+    pts = N * N
+    per = []
+    for i, (name, shape, r, t) in enumerate(suite):
+        ms = float(np.mean([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(a.steps)]))
+        upd = pts * t
+        byts = 2 * 4 * pts  # each point read once and written once per launch
+        taps_n = (4 * r + 1) if shape == "star" else (2 * r + 1) ** 2
+        per.append(dict(name=name, ms=round(ms, 4), gpts=round(upd / ms / 1e6, 2),
+                        gbps=round(byts / ms / 1e6, 1),
+                        hbm_frac=round(byts / ms / 1e6 / HBM_PEAK_GBPS, 3),
+                        gflops=round(upd * (2 * taps_n - 1) / ms / 1e6, 1)))
+    updates_per_step = sum(pts * t for (_, _, _, t) in suite)
+    value = updates_per_step / (ms_per_step * 1e-3) / 1e9
+
+    dom = max(per, key=lambda p: p["ms"])
+    dom_bytes = 2 * 4 * pts
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{dom['name']}_8192_f32.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = dict(bound="hbm", kernel=f"{dom['name']} f32 8192^2",
+                    achieved=round(dom_bytes / (dom["ms"] * 1e-3) / 1e9, 1), peak=HBM_PEAK_GBPS,
+                    unit="GB/s", frac=round(dom_bytes / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                    traffic=traffic, algorithmic_bytes=dom_bytes)
+
+    # ---------------------------------------------------------------- stream bandwidth
+    cp_a = torch.empty(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+    cp_b = torch.empty_like(cp_a)
+    api.fill_uniform(cp_a, 1)
+    for _ in range(2):
+        api.stream_copy(cp_a, cp_b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(5):
+        api.stream_copy(cp_a, cp_b)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbps = 2 * cp_a.numel() * 4 * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del cp_a, cp_b
+    for p in per:
+        p["stream_frac"] = round(p["gbps"] / copy_gbps, 3)
+
+    extras = {}
+    if not a.no_extras:
+        extras = side_runs(suite, taps, stream, per)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(suite, taps)
+
+    line = {
+        "metric": "Gpoints/s + % HBM roofline (1 GPU); BF16-storage vs FP32 plain at stated err",
+        "value": round(value, 2),
+        "unit": "Gpoints/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: U[-1,1) grid from splitmix64 seed 0, taps U[0,1) normalised to 1",
+        "config": {
+            "workload": "2D SSAM suite on 8192x8192 FP32: box r1-5 (9/25/49/81/121 pt), "
+                        "star r3-5 (13/17/21 pt), temporal 5-pt and 9-pt star T=8; "
+                        "one sweep of each per step",
+            "grid": [N, N],
+            "kernels": [s[0] for s in suite],
+            "point_updates_per_step": updates_per_step,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stream_copy_gbps": round(copy_gbps, 1),
+        "per_kernel": per,
+        **extras,
+        "device": torch.cuda.get_device_name(0),
+    }
+    print(json.dumps(line))
+
+
+def side_runs(suite, taps, stream, per_f32):
+    """FP64 and BF16-storage versions of the suite (same input values), timed with
+    events; BF16 errors against the GPU FP64 results."""
+    import numpy as np
+    import torch
+
+    from ssam_amd import _lib as L
+    from ssam_amd import api
+
+    out = {}
+    n = N
+    x64 = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    api.fill_uniform(x64, 0, -1.0, 1.0)
+    res = {}
+    for label, dt in (("fp64", torch.float64), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
+        src = x64.to(dt)
+        dst = torch.empty_like(src)
+        kern = []
+        for i, (name, shape, r, t) in enumerate(suite):
+            f = taps(shape, r)
+            sh = L.STAR if shape == "star" else L.BOX
+            api.stencil2d(src, dst, f, shape=sh, steps=t, stream=stream)  # warm
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(3):
+                api.stencil2d(src, dst, f, shape=sh, steps=t, stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 3
+            entry = dict(name=name, ms=round(ms, 4), gpts=round(n * n * t / ms / 1e6, 2),
+                         speedup_vs_f32=round(per_f32[i]["ms"] / ms, 3))
+            if label == "fp64":
+                res[name] = dst.clone()
+            else:
+                ref = res[name]
+                d = (dst.to(torch.float64) - ref).abs()
+                denom = ref.abs().max()
+                entry["max_abs_err"] = float(d.max())
+                entry["rms_err"] = float(torch.sqrt((d * d).mean()))
+                entry["max_rel_err_normwise"] = float(d.max() / denom)
+            kern.append(entry)
+        upd = sum(n * n * t for (_, _, _, t) in suite)
+        tot = sum(k["ms"] for k in kern)
+        out[label] = dict(value_gpts=round(upd / tot / 1e6, 2), ms_per_step=round(tot, 4),
+                          kernels=kern)
+        del src, dst
+    return out
+
+
+def cpu_baseline(suite, taps):
+    """The oracle (restated reference gold, OpenMP over rows) on a bounded slab of
+    the same workload: 64 rows x 8192 columns per kernel."""
+    import numpy as np
+
+    from oracle import oracle
+    from ssam_amd import workloads as wl
+
+    rows = 64
+    x = wl.uniform(0, rows * N, -1.0, 1.0).astype(np.float32).reshape(rows, N)
+    upd, sec = 0, 0.0
+    for name, shape, r, t in suite:
+        f = taps(shape, r).astype(np.float32)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            if t > 1:
+                oracle.conv2d_iter(x, f, t)
+            else:
+                oracle.conv2d(x, f)
+            times.append(time.perf_counter() - t0)
+        sec += float(np.median(times))
+        upd += rows * N * t
+    return dict(value=round(upd / sec / 1e9, 4), unit="Gpoints/s", cores=oracle.max_threads(),
+                kind="port",
+                sample=f"same 10-kernel FP32 suite on a {rows}x{N} slab (one sweep each; "
+                       "temporal kernels iterated 8x), median of 3")
+
+
+if __name__ == "__main__":
+    main()

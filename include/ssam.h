@@ -1,0 +1,204 @@
+/*
+ * ssam.h -- C ABI of libssam.so, the MI355X (gfx950) SSAM stencil / convolution
+ * library.
+ *
+ * Conventions (all entry points):
+ *   - device pointers are caller-owned HIP device memory; the library never
+ *     allocates grids and never aliases src and dst (overlap -> SSAM_E_INVALID);
+ *   - sizes and pitches are in elements; pitch >= width;
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream); every
+ *     call is asynchronous on that stream unless its comment says otherwise;
+ *   - the return value is an ssam_status; the library never aborts or throws.
+ * The reference (peng-codes/SSAM) has no library API: its callers are the C++
+ * harness entry points `int stencil_<pts>_<dtype>(int argc, char** argv)`
+ * (ssai-2dstencil/main.cpp:29-53, ssai-3dstencil/main.cpp:5-9) launching the
+ * templated kernels cited per entry below.  The drivers under
+ * ssam_amd/drivers/ keep those command lines and call these functions.
+ */
+#ifndef SSAM_H
+#define SSAM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum {
+    SSAM_OK = 0,
+    SSAM_E_INVALID = 1,     /* bad argument: null pointer, size, shape, aliasing */
+    SSAM_E_UNSUPPORTED = 2, /* well-formed request this build has no kernel for */
+    SSAM_E_HIP = 3,         /* HIP runtime reported an error (launch, device) */
+    SSAM_E_NOMEM = 4        /* host-side staging allocation failed */
+} ssam_status;
+
+/* ---------------------------------------------------------------- enums */
+typedef enum {
+    SSAM_F32 = 0,  /* FP32 storage, FP32 accumulation */
+    SSAM_F64 = 1,  /* FP64 storage, FP64 accumulation */
+    SSAM_BF16 = 2, /* BF16 storage, FP32 accumulation and running state */
+    SSAM_F16 = 3   /* FP16 storage, FP32 accumulation and running state */
+} ssam_dtype;
+
+typedef enum {
+    SSAM_SHAPE_STAR = 0,     /* 2D: centre row + centre column of a (2r+1)^2 box;
+                                3D: the 6r+1-point axis star */
+    SSAM_SHAPE_BOX = 1,      /* 2D: full (2r+1)^2 box */
+    SSAM_SHAPE_DENSE = 2,    /* 2D: any fw x fh filter (even sizes allowed) */
+    SSAM_SHAPE_FACE_EDGE = 3 /* 3D: 19 points = centre, 6 faces, 12 edges */
+} ssam_shape;
+
+typedef enum {
+    SSAM_BOUNDARY_ZERO = 0,      /* reads outside the grid are 0 (reference CPU gold,
+                                    ssai-2dconv/common.h:200-206, and conv2D.cu:36-41) */
+    SSAM_BOUNDARY_CLAMP = 1,     /* reads clamp to the edge (reference GPU stencils,
+                                    stencil2D_5pt-v1.cu:29-46); steps == 1 only */
+    SSAM_BOUNDARY_HOLD = 2,      /* cells within r of the border keep their input
+                                    value; steps == 1 only */
+    SSAM_BOUNDARY_DIRICHLET0 = 3 /* 3D: the outer r layers of every output are 0
+                                    (reference 3D gold, tmp3d7pt-v2.cu:284-296) */
+} ssam_boundary;
+
+#define SSAM_FLAG_COMPENSATED 1u /* compensated (Neumaier) combination of the
+                                    per-filter-row partial sums */
+
+/* ---------------------------------------------------------------- descriptors */
+typedef struct {
+    int32_t storage;      /* ssam_dtype of src, dst and work */
+    int32_t shape;        /* STAR, BOX or DENSE */
+    int32_t fw, fh;       /* filter width / height; STAR and BOX: fw == fh == 2r+1 */
+    int32_t steps;        /* time steps applied (>= 1); > 1 only for ZERO boundary */
+    int32_t boundary;     /* ssam_boundary: ZERO, CLAMP or HOLD */
+    uint32_t flags;       /* SSAM_FLAG_* */
+    const double* coeffs; /* HOST memory, fh*fw taps row-major: w[m*fw + n] weights
+                             in(x + n - fw/2, y + m - fh/2) (correlation, no flip).
+                             STAR reads only the centre row and column. */
+} ssam_stencil2d_desc;
+
+typedef struct {
+    int32_t storage;      /* ssam_dtype (F32 or F64) */
+    int32_t shape;        /* STAR (radius 1 or 2) or FACE_EDGE */
+    int32_t radius;       /* 1 or 2 for STAR, 1 for FACE_EDGE */
+    int32_t steps;        /* sweeps (>= 1) */
+    int32_t boundary;     /* SSAM_BOUNDARY_DIRICHLET0 */
+    uint32_t flags;
+    const double* coeffs; /* HOST memory.  STAR r: 6r+1 taps ordered
+                             c, x-1, x+1, y-1, y+1, z-1, z+1, x-2, x+2, y-2, y+2, z-2, z+2.
+                             FACE_EDGE: 19 taps ordered c, 6 faces (as STAR r=1), then
+                             edges (y-1,z-1) (y+1,z-1) (y-1,z+1) (y+1,z+1)
+                             (x-1,z-1) (x+1,z-1) (x-1,z+1) (x+1,z+1)
+                             (x-1,y-1) (x+1,y-1) (x-1,y+1) (x+1,y+1). */
+} ssam_stencil3d_desc;
+
+/* ---------------------------------------------------------------- library */
+const char* ssam_version(void);
+const char* ssam_strerror(int status);
+
+/* ---------------------------------------------------------------- 2D generic
+ * Applies `steps` sweeps of the 2D stencil / correlation described by `desc` to
+ * src (width x height, row pitch `pitch`), result in dst.  Temporal steps are
+ * fused in registers up to 8 per launch; a `steps` needing more than one launch
+ * ping-pongs through `work` (same shape as dst; may be NULL otherwise).
+ * Replaces the kernel launches of stencil2D_5pt-v1.cu:194-195, stencil2D_*pt.cu,
+ * stencil2d_temp_{5,9}pt.cu:146-147 and conv2D.cu:128-131. */
+int ssam_stencil2d(const ssam_stencil2d_desc* desc, const void* src, void* dst, int64_t width,
+                   int64_t height, int64_t pitch, void* work, void* stream);
+
+/* ---------------------------------------------------------------- 2D, reference kernels
+ * stencil2d_v1::j2d5pt(src, dst, width, height, fc, fn, fs, fw, fe)
+ * (ssai-2dstencil/stencil2D_5pt-v1.cu:8-9); zero boundary, pitch = width. */
+int ssam_j2d5pt_f32(const float* src, float* dst, int width, int height, float fc, float fn,
+                    float fs, float fw, float fe, void* stream);
+int ssam_j2d5pt_f64(const double* src, double* dst, int width, int height, double fc, double fn,
+                    double fs, double fw, double fe, void* stream);
+
+/* Star of radius r = 1..6 (j2d9pt .. j2d25pt, stencil2D_{9,13,17,21,25}pt.cu:9-11).
+ * coefs[4r+1] in the reference kernels' argument order:
+ *   fc, fn0, fs0, fw0, fe0, fn1, fs1, fw1, fe1, ...  (fnk = k+1 rows north). */
+int ssam_j2dstar_f32(const float* src, float* dst, int width, int height, int radius,
+                     const float* coefs, void* stream);
+int ssam_j2dstar_f64(const double* src, double* dst, int width, int height, int radius,
+                     const double* coefs, void* stream);
+
+/* Box of radius r (j2d64pt / j2d81pt / j2d121pt, stencil2D_64pt.cu:74,
+ * stencil2D_81pt.cu:92, stencil2D_121pt.cu:132, which bake their taps in as
+ * macros; here they are a host array w[(2r+1)^2], row-major). */
+int ssam_j2dbox_f32(const float* src, float* dst, int width, int height, int radius,
+                    const float* w, void* stream);
+int ssam_j2dbox_f64(const double* src, double* dst, int width, int height, int radius,
+                    const double* w, void* stream);
+
+/* Temporally blocked 5-point (stencil2d_tmp::j2d5pt, stencil2d_temp_5pt.cu:9-10) and
+ * 9-point star (stencil2d_tmp::j2d9pt, stencil2d_temp_9pt.cu:12-14): `steps`
+ * zero-padded sweeps; `work` as for ssam_stencil2d. */
+int ssam_j2d5pt_temporal_f32(const float* src, float* dst, int width, int height, float fc,
+                             float fn, float fs, float fw, float fe, int steps, float* work,
+                             void* stream);
+int ssam_j2d5pt_temporal_f64(const double* src, double* dst, int width, int height, double fc,
+                             double fn, double fs, double fw, double fe, int steps,
+                             double* work, void* stream);
+int ssam_j2d9pt_temporal_f32(const float* src, float* dst, int width, int height, float fc,
+                             float fn0, float fs0, float fw0, float fe0, float fn1, float fs1,
+                             float fw1, float fe1, int steps, float* work, void* stream);
+int ssam_j2d9pt_temporal_f64(const double* src, double* dst, int width, int height, double fc,
+                             double fn0, double fs0, double fw0, double fe0, double fn1,
+                             double fs1, double fw1, double fe1, int steps, double* work,
+                             void* stream);
+
+/* CONV2D::kernel_convolution2D(src, dst, width, widthStride, height, weight)
+ * (ssai-2dconv/conv2D.cu:9-12): FW x FH correlation, zero padding, pitched rows.
+ * `weight` is DEVICE memory (fh*fw, row-major) as in the reference; it is read
+ * back once per call, so this entry SYNCHRONISES `stream`.  Use ssam_stencil2d
+ * with SSAM_SHAPE_DENSE for an asynchronous, graph-capturable call. */
+int ssam_conv2d_f32(const float* src, float* dst, int width, int width_stride, int height,
+                    const float* weight, int fw, int fh, void* stream);
+int ssam_conv2d_f64(const double* src, double* dst, int width, int width_stride, int height,
+                    const double* weight, int fw, int fh, void* stream);
+
+/* ---------------------------------------------------------------- 3D
+ * `steps` sweeps of a 3D stencil on a dense nx*ny*nz grid (index z*ny*nx + y*nx + x,
+ * tmp3d7pt-v2.cu:107), F32 or F64.  Every output point is written: the outer r
+ * layers are 0 (SSAM_BOUNDARY_DIRICHLET0), the interior gets the stencil of the
+ * input.  steps > 1 ping-pongs through `work` (same size; may be NULL otherwise).
+ * Replaces kernel3d_3d7pt (tmp3d7pt-v2.cu:62-64), kernel3d_3d13pt
+ * (tmp3d13pt.cu:74-76), kernel3d_poisson (tmpPoisson.cu:86-88). */
+int ssam_stencil3d(const ssam_stencil3d_desc* desc, const void* src, void* dst, int64_t nx,
+                   int64_t ny, int64_t nz, void* work, void* stream);
+
+/* Reference operators with their reference coefficients (tmp3d7pt-v2.cu:9-15,
+ * tmp3d13pt.cu:9-21, tmpPoisson.cu:9-27), `iters` sweeps. */
+int ssam_j3d7pt_f32(const float* src, float* dst, int n, int iters, float* work, void* stream);
+int ssam_j3d7pt_f64(const double* src, double* dst, int n, int iters, double* work, void* stream);
+int ssam_j3d13pt_f32(const float* src, float* dst, int n, int iters, float* work, void* stream);
+int ssam_j3d13pt_f64(const double* src, double* dst, int n, int iters, double* work,
+                     void* stream);
+int ssam_poisson_f32(const float* src, float* dst, int n, int iters, float* work, void* stream);
+int ssam_poisson_f64(const double* src, double* dst, int n, int iters, double* work,
+                     void* stream);
+
+/* 19-point Jacobi sweeps for the 3D Laplace problem (this library's operator; the
+ * reference's "poisson" grows without bound when iterated):
+ *   u'(c) = (2*sum_faces u + sum_edges u) / 24, zero Dirichlet boundary,
+ * running state in the storage type (F32 or F64, never BF16).  If residual_l2 is
+ * not NULL it receives the FP64 L2 norm of r = -A u of the input of the LAST
+ * sweep, A u = (24 u - 2 faces - edges)/6, reduced on the GPU in a fixed order;
+ * this entry then synchronises `stream`. */
+int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz,
+                  int steps, void* work, double* residual_l2, void* stream);
+
+/* ---------------------------------------------------------------- utilities
+ * Seeded synthetic input: element i = a + (b-a) * (splitmix64_{seed}(offset+i) >> 40)
+ * * 2^-24, written in `dtype`.  Same values as the CPU generator of the oracle. */
+int ssam_fill_uniform(void* dst, int dtype, int64_t n, uint64_t seed, uint64_t offset, double a,
+                      double b, void* stream);
+
+/* Device copy of `bytes` (multiple of 16, 16-byte aligned) with dwordx4 loads and
+ * stores: the measured HBM stream bandwidth the roofline fractions refer to. */
+int ssam_stream_copy(const void* src, void* dst, int64_t bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSAM_H */

@@ -1,0 +1,214 @@
+"""Host-side operators over libssam.so for torch device tensors.
+
+Mirrors the reference harness operators (ssai-2dstencil/stencil2D_*.cu `Test*`,
+ssai-2dconv/conv2D.cu `Test_conv2D`, ssai-3dstencil/tmp3d*.cu `_3d*`): the caller
+owns the device buffers, src and dst never alias, and every call goes through the
+C ABI of include/ssam.h.  Errors raise `SsamError` with the library's status; there
+is no CPU fallback.  torch is used only for device memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import check
+
+_DTYPES = {torch.float32: L.F32, torch.float64: L.F64, torch.bfloat16: L.BF16,
+           torch.float16: L.F16}
+
+
+def dtype_code(t: torch.dtype) -> int:
+    try:
+        return _DTYPES[t]
+    except KeyError:
+        raise TypeError(f"unsupported storage dtype {t}") from None
+
+
+def _stream(stream: Optional[torch.cuda.Stream]) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _dev(t: Optional[torch.Tensor], name: str) -> ctypes.c_void_p:
+    if t is None:
+        return ctypes.c_void_p(0)
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor (got {t.device})")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_2d(src: torch.Tensor, dst: torch.Tensor) -> tuple[int, int, int]:
+    if src.dim() != 2 or dst.dim() != 2 or src.shape != dst.shape:
+        raise ValueError("src and dst must be 2-D tensors of one shape")
+    if src.dtype != dst.dtype:
+        raise ValueError("src and dst must share the storage dtype")
+    if src.stride(1) != 1 or dst.stride(1) != 1 or src.stride(0) != dst.stride(0):
+        raise ValueError("rows must be contiguous with one common pitch")
+    h, w = src.shape
+    return w, h, src.stride(0)
+
+
+def stencil2d(src: torch.Tensor, dst: torch.Tensor, coeffs, shape: int = L.BOX, steps: int = 1,
+              boundary: int = L.ZERO, work: Optional[torch.Tensor] = None, flags: int = 0,
+              stream: Optional[torch.cuda.Stream] = None) -> None:
+    """`steps` sweeps of the fh x fw correlation `coeffs` (row-major, taps weight
+    in(x + n - fw/2, y + m - fh/2)) over src -> dst (ssam_stencil2d)."""
+    w, h, pitch = _check_2d(src, dst)
+    c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
+    if c.ndim != 2:
+        raise ValueError("coeffs must be a 2-D (fh, fw) array")
+    d = L.Stencil2DDesc()
+    d.storage = dtype_code(src.dtype)
+    d.shape = shape
+    d.fh, d.fw = c.shape
+    d.steps = steps
+    d.boundary = boundary
+    d.flags = flags
+    d.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = L.lib().ssam_stencil2d(ctypes.byref(d), _dev(src, "src"), _dev(dst, "dst"), w, h, pitch,
+                                _dev(work, "work"), _stream(stream))
+    check(rc, "ssam_stencil2d")
+
+
+def _sfx(t: torch.Tensor) -> str:
+    if t.dtype == torch.float32:
+        return "f32"
+    if t.dtype == torch.float64:
+        return "f64"
+    raise TypeError("the reference-kernel entry points take float32 or float64")
+
+
+def j2d5pt(src, dst, fc, fn, fs, fw, fe, stream=None) -> None:
+    """stencil2d_v1::j2d5pt (stencil2D_5pt-v1.cu:8-9) via ssam_j2d5pt_*."""
+    w, h, pitch = _check_2d(src, dst)
+    if pitch != w:
+        raise ValueError("j2d5pt takes unpitched grids (pitch == width)")
+    fn_ = getattr(L.lib(), f"ssam_j2d5pt_{_sfx(src)}")
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, h, fc, fn, fs, fw, fe, _stream(stream)),
+          "ssam_j2d5pt")
+
+
+def j2dstar(src, dst, radius: int, coefs: Sequence[float], stream=None) -> None:
+    """j2d9pt..j2d25pt: coefs = fc, fn0, fs0, fw0, fe0, fn1, ... (4r+1 values)."""
+    w, h, pitch = _check_2d(src, dst)
+    if pitch != w:
+        raise ValueError("j2dstar takes unpitched grids (pitch == width)")
+    ct = np.ascontiguousarray(np.asarray(coefs, dtype=np.float32 if src.dtype == torch.float32
+                                         else np.float64))
+    if ct.size != 4 * radius + 1:
+        raise ValueError("a star of radius r takes 4r+1 coefficients")
+    fn_ = getattr(L.lib(), f"ssam_j2dstar_{_sfx(src)}")
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, h, radius, ct.ctypes.data_as(ctypes.c_void_p),
+              _stream(stream)), "ssam_j2dstar")
+
+
+def j2dbox(src, dst, radius: int, wts, stream=None) -> None:
+    """j2d64pt / j2d81pt / j2d121pt with run-time taps w[(2r+1)^2]."""
+    w, h, pitch = _check_2d(src, dst)
+    if pitch != w:
+        raise ValueError("j2dbox takes unpitched grids (pitch == width)")
+    ct = np.ascontiguousarray(np.asarray(wts, dtype=np.float32 if src.dtype == torch.float32
+                                         else np.float64)).ravel()
+    if ct.size != (2 * radius + 1) ** 2:
+        raise ValueError("a box of radius r takes (2r+1)^2 taps")
+    fn_ = getattr(L.lib(), f"ssam_j2dbox_{_sfx(src)}")
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, h, radius, ct.ctypes.data_as(ctypes.c_void_p),
+              _stream(stream)), "ssam_j2dbox")
+
+
+def j2d5pt_temporal(src, dst, fc, fn, fs, fw, fe, steps: int, work=None, stream=None) -> None:
+    """stencil2d_tmp::j2d5pt (stencil2d_temp_5pt.cu:9-10): `steps` fused sweeps."""
+    w, h, pitch = _check_2d(src, dst)
+    fn_ = getattr(L.lib(), f"ssam_j2d5pt_temporal_{_sfx(src)}")
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, h, fc, fn, fs, fw, fe, steps,
+              _dev(work, "work"), _stream(stream)), "ssam_j2d5pt_temporal")
+
+
+def j2d9pt_temporal(src, dst, c9: Sequence[float], steps: int, work=None, stream=None) -> None:
+    """stencil2d_tmp::j2d9pt (stencil2d_temp_9pt.cu:12-14); c9 = fc, fn0, fs0, fw0,
+    fe0, fn1, fs1, fw1, fe1."""
+    w, h, pitch = _check_2d(src, dst)
+    fn_ = getattr(L.lib(), f"ssam_j2d9pt_temporal_{_sfx(src)}")
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, h, *[float(v) for v in c9], steps,
+              _dev(work, "work"), _stream(stream)), "ssam_j2d9pt_temporal")
+
+
+def conv2d(src, dst, weight: torch.Tensor, stream=None) -> None:
+    """CONV2D::kernel_convolution2D (conv2D.cu:9-12): pitched rows, DEVICE weights
+    (fh, fw); synchronises the stream to read the weights."""
+    w, h, pitch = _check_2d(src, dst)
+    if weight.dim() != 2 or weight.dtype != src.dtype:
+        raise ValueError("weight must be a 2-D device tensor of the grid dtype")
+    fh, fw = weight.shape
+    fn_ = getattr(L.lib(), f"ssam_conv2d_{_sfx(src)}")
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, pitch, h, _dev(weight.contiguous(), "weight"),
+              fw, fh, _stream(stream)), "ssam_conv2d")
+
+
+def _check_3d(src, dst) -> tuple[int, int, int]:
+    if src.dim() != 3 or src.shape != dst.shape or not src.is_contiguous() or not dst.is_contiguous():
+        raise ValueError("src and dst must be contiguous 3-D tensors (nz, ny, nx) of one shape")
+    if src.dtype != dst.dtype:
+        raise ValueError("src and dst must share the dtype")
+    nz, ny, nx = src.shape
+    return nx, ny, nz
+
+
+def stencil3d(src, dst, coeffs, shape: int = L.STAR, radius: int = 1, steps: int = 1,
+              work=None, stream=None) -> None:
+    """ssam_stencil3d: `steps` sweeps, zero Dirichlet output boundary."""
+    nx, ny, nz = _check_3d(src, dst)
+    c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
+    d = L.Stencil3DDesc()
+    d.storage = dtype_code(src.dtype)
+    d.shape = shape
+    d.radius = radius
+    d.steps = steps
+    d.boundary = L.DIRICHLET0
+    d.flags = 0
+    d.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    check(L.lib().ssam_stencil3d(ctypes.byref(d), _dev(src, "src"), _dev(dst, "dst"), nx, ny, nz,
+                                 _dev(work, "work"), _stream(stream)), "ssam_stencil3d")
+
+
+def ref3d(name: str, src, dst, iters: int = 1, work=None, stream=None) -> None:
+    """Reference 3D operators with reference coefficients: name in
+    {'j3d7pt', 'j3d13pt', 'poisson'}; cubic grids."""
+    nx, ny, nz = _check_3d(src, dst)
+    if not nx == ny == nz:
+        raise ValueError("the reference 3D operators take cubic grids")
+    fn_ = getattr(L.lib(), f"ssam_{name}_{_sfx(src)}")
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), nx, iters, _dev(work, "work"), _stream(stream)),
+          f"ssam_{name}")
+
+
+def jacobi19(src, dst, steps: int, work=None, residual: bool = True, stream=None) -> Optional[float]:
+    """19-point Jacobi sweeps (zero Dirichlet); returns the FP64 residual L2 norm of
+    the last sweep's input (synchronises the stream) when `residual`."""
+    nx, ny, nz = _check_3d(src, dst)
+    r = ctypes.c_double(0.0)
+    check(L.lib().ssam_jacobi19(dtype_code(src.dtype), _dev(src, "src"), _dev(dst, "dst"), nx, ny,
+                                nz, steps, _dev(work, "work"),
+                                ctypes.byref(r) if residual else None, _stream(stream)),
+          "ssam_jacobi19")
+    return r.value if residual else None
+
+
+def fill_uniform(t: torch.Tensor, seed: int, a: float = 0.0, b: float = 1.0, offset: int = 0,
+                 stream=None) -> torch.Tensor:
+    """Seeded U[a,b) fill on the device (splitmix64, same values as the CPU oracle)."""
+    if not t.is_contiguous():
+        raise ValueError("fill_uniform needs a contiguous tensor")
+    check(L.lib().ssam_fill_uniform(_dev(t, "t"), dtype_code(t.dtype), t.numel(), seed, offset,
+                                    a, b, _stream(stream)), "ssam_fill_uniform")
+    return t
+
+
+def stream_copy(src: torch.Tensor, dst: torch.Tensor, stream=None) -> None:
+    nbytes = src.numel() * src.element_size()
+    check(L.lib().ssam_stream_copy(_dev(src, "src"), _dev(dst, "dst"), nbytes, _stream(stream)),
+          "ssam_stream_copy")

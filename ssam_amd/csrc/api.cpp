@@ -1,0 +1,485 @@
+// api.cpp -- the C ABI of libssam.so (include/ssam.h): argument validation,
+// status codes, the mapping of the reference kernels' entry points onto the
+// generic 2D/3D paths, and launch sequencing (temporal chunks, ping-pong).
+// Never aborts or throws: every failure is a status code.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "ssam_device.hpp"
+#include "ssam_internal.hpp"
+
+namespace ssam {
+namespace {
+
+constexpr int kMaxStarRadius = 6;
+constexpr int kMaxBoxRadius = 9;
+constexpr int kMaxFusedSteps = 8;
+
+int elem_size(int dtype) {
+    switch (dtype) {
+        case SSAM_F32: return 4;
+        case SSAM_F64: return 8;
+        case SSAM_BF16:
+        case SSAM_F16: return 2;
+        default: return 0;
+    }
+}
+
+bool overlaps(const void* a, int64_t abytes, const void* b, int64_t bbytes) {
+    const char* pa = static_cast<const char*>(a);
+    const char* pb = static_cast<const char*>(b);
+    return pa < pb + bbytes && pb < pa + abytes;
+}
+
+int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return std::atoi(v);
+}
+
+int launch2d(int storage, const Req2D& rq, hipStream_t st) {
+    switch (storage) {
+        case SSAM_F32: return dispatch2d<float>(rq, st);
+        case SSAM_F64: return dispatch2d<double>(rq, st);
+        case SSAM_BF16: return dispatch2d<bf16_t>(rq, st);
+        case SSAM_F16: return dispatch2d<f16_t>(rq, st);
+        default: return SSAM_E_INVALID;
+    }
+}
+
+// Largest fused step count the kernels provide for this shape.
+int max_fused(int shape, int radius) {
+    return (shape == SSAM_SHAPE_STAR && radius <= 2) ? kMaxFusedSteps : 1;
+}
+
+}  // namespace
+
+int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int64_t width,
+                   int64_t height, int64_t pitch, void* work, hipStream_t st) {
+    if (!d || !src || !dst || !d->coeffs) return SSAM_E_INVALID;
+    const int es = elem_size(d->storage);
+    if (!es) return SSAM_E_INVALID;
+    if (width < 1 || height < 1 || pitch < width) return SSAM_E_INVALID;
+    if (width > (1LL << 30) || height > (1LL << 30) || pitch > (1LL << 31))
+        return SSAM_E_UNSUPPORTED;
+    if (d->steps < 1) return SSAM_E_INVALID;
+    if (d->boundary != SSAM_BOUNDARY_ZERO && d->boundary != SSAM_BOUNDARY_CLAMP &&
+        d->boundary != SSAM_BOUNDARY_HOLD)
+        return SSAM_E_INVALID;
+    if (d->boundary != SSAM_BOUNDARY_ZERO && d->steps != 1) return SSAM_E_INVALID;
+    if (d->fw < 1 || d->fh < 1) return SSAM_E_INVALID;
+
+    // Embed the filter in a (2R+1)^2 box of taps centred like the reference's
+    // correlation (offset n - fw/2, m - fh/2; common.h:196-197).
+    int shape = d->shape;
+    int R = 0;
+    if (shape == SSAM_SHAPE_STAR || shape == SSAM_SHAPE_BOX) {
+        if (d->fw != d->fh || d->fw % 2 == 0 || d->fw < 3) return SSAM_E_INVALID;
+        R = d->fw / 2;
+        if (R > (shape == SSAM_SHAPE_STAR ? kMaxStarRadius : kMaxBoxRadius))
+            return SSAM_E_UNSUPPORTED;
+    } else if (shape == SSAM_SHAPE_DENSE) {
+        R = d->fw / 2 > d->fh / 2 ? d->fw / 2 : d->fh / 2;
+        if (R < 1) R = 1;
+        if (R > kMaxBoxRadius) return SSAM_E_UNSUPPORTED;
+    } else {
+        return SSAM_E_INVALID;
+    }
+    const int K = 2 * R + 1;
+    std::vector<double> taps(static_cast<size_t>(K) * K, 0.0);
+    if (shape == SSAM_SHAPE_DENSE) {
+        for (int m = 0; m < d->fh; ++m)
+            for (int n = 0; n < d->fw; ++n) {
+                const int dy = m - d->fh / 2, dx = n - d->fw / 2;
+                taps[static_cast<size_t>(dy + R) * K + (dx + R)] = d->coeffs[m * d->fw + n];
+            }
+    } else {
+        for (int i = 0; i < K * K; ++i) taps[i] = d->coeffs[i];
+        if (shape == SSAM_SHAPE_STAR) {
+            for (int m = 0; m < K; ++m)
+                for (int n = 0; n < K; ++n)
+                    if (m != R && n != R) taps[static_cast<size_t>(m) * K + n] = 0.0;
+        }
+    }
+    const int kshape = shape == SSAM_SHAPE_STAR ? 0 : 1;  // DENSE runs as a box
+
+    const int64_t bytes = ((height - 1) * pitch + width) * es;
+    if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
+
+    // split the steps into fused launches (8, 4, 2, 1) chained through work/dst
+    const int fmax = max_fused(shape, R);
+    std::vector<int> chunks;
+    for (int left = d->steps; left > 0;) {
+        int t = fmax;
+        while (t > left) t >>= 1;
+        chunks.push_back(t);
+        left -= t;
+    }
+    if (chunks.size() > 1) {
+        if (!work) return SSAM_E_INVALID;
+        if (overlaps(work, bytes, dst, bytes) || overlaps(work, bytes, src, bytes))
+            return SSAM_E_INVALID;
+    }
+
+    Req2D rq;
+    rq.width = width;
+    rq.height = height;
+    rq.pitch = pitch;
+    rq.shape = kshape;
+    rq.radius = R;
+    rq.boundary = d->boundary;
+    rq.taps = taps.data();
+    rq.seg_rows = env_int("SSAM_SEG_ROWS", 0);
+    rq.waves_per_simd = env_int("SSAM_WAVES_PER_SIMD", 4);
+    const size_t n = chunks.size();
+    const void* in = src;
+    for (size_t i = 0; i < n; ++i) {
+        // the last launch writes dst; earlier ones alternate so that holds
+        void* out = ((n - 1 - i) % 2 == 0) ? dst : work;
+        rq.src = in;
+        rq.dst = out;
+        rq.steps = chunks[i];
+        const int rc = launch2d(d->storage, rq, st);
+        if (rc != SSAM_OK) return rc;
+        in = out;
+    }
+    return SSAM_OK;
+}
+
+}  // namespace ssam
+
+using namespace ssam;
+
+extern "C" {
+
+const char* ssam_version(void) { return "ssam-mi355x 0.1.0 (gfx950)"; }
+
+const char* ssam_strerror(int status) {
+    switch (status) {
+        case SSAM_OK: return "success";
+        case SSAM_E_INVALID: return "invalid argument";
+        case SSAM_E_UNSUPPORTED: return "unsupported request";
+        case SSAM_E_HIP: return "HIP runtime error";
+        case SSAM_E_NOMEM: return "out of host memory";
+        default: return "unknown status";
+    }
+}
+
+int ssam_stencil2d(const ssam_stencil2d_desc* desc, const void* src, void* dst, int64_t width,
+                   int64_t height, int64_t pitch, void* work, void* stream) {
+    return stencil2d_impl(desc, src, dst, width, height, pitch, work,
+                          static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ reference entries
+namespace {
+
+template <typename T>
+int star_call(int dtype, const T* src, T* dst, int width, int height, int radius, const T* c,
+              int steps, T* work, void* stream) {
+    if (!c) return SSAM_E_INVALID;
+    if (radius < 1 || radius > kMaxStarRadius) return SSAM_E_UNSUPPORTED;
+    const int K = 2 * radius + 1, R = radius;
+    std::vector<double> w(static_cast<size_t>(K) * K, 0.0);
+    // fc, then per ring k: fn_k, fs_k, fw_k, fe_k (k+1 cells from the centre)
+    w[R * K + R] = c[0];
+    for (int k = 0; k < R; ++k) {
+        w[(R - 1 - k) * K + R] = c[1 + 4 * k + 0];
+        w[(R + 1 + k) * K + R] = c[1 + 4 * k + 1];
+        w[R * K + (R - 1 - k)] = c[1 + 4 * k + 2];
+        w[R * K + (R + 1 + k)] = c[1 + 4 * k + 3];
+    }
+    ssam_stencil2d_desc d;
+    d.storage = dtype;
+    d.shape = SSAM_SHAPE_STAR;
+    d.fw = d.fh = K;
+    d.steps = steps;
+    d.boundary = SSAM_BOUNDARY_ZERO;
+    d.flags = 0;
+    d.coeffs = w.data();
+    return ssam_stencil2d(&d, src, dst, width, height, width, work, stream);
+}
+
+template <typename T>
+int box_call(int dtype, const T* src, T* dst, int width, int height, int radius, const T* wt,
+             void* stream) {
+    if (!wt) return SSAM_E_INVALID;
+    if (radius < 1 || radius > kMaxBoxRadius) return SSAM_E_UNSUPPORTED;
+    const int K = 2 * radius + 1;
+    std::vector<double> w(static_cast<size_t>(K) * K);
+    for (int i = 0; i < K * K; ++i) w[i] = wt[i];
+    ssam_stencil2d_desc d;
+    d.storage = dtype;
+    d.shape = SSAM_SHAPE_BOX;
+    d.fw = d.fh = K;
+    d.steps = 1;
+    d.boundary = SSAM_BOUNDARY_ZERO;
+    d.flags = 0;
+    d.coeffs = w.data();
+    return ssam_stencil2d(&d, src, dst, width, height, width, nullptr, stream);
+}
+
+template <typename T>
+int conv_call(int dtype, const T* src, T* dst, int width, int width_stride, int height,
+              const T* weight, int fw, int fh, void* stream) {
+    if (!weight || fw < 1 || fh < 1) return SSAM_E_INVALID;
+    if (fw > 2 * kMaxBoxRadius + 1 || fh > 2 * kMaxBoxRadius + 1) return SSAM_E_UNSUPPORTED;
+    std::vector<T> hw(static_cast<size_t>(fw) * fh);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(hw.data(), weight, hw.size() * sizeof(T), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return SSAM_E_HIP;
+    std::vector<double> w(hw.begin(), hw.end());
+    ssam_stencil2d_desc d;
+    d.storage = dtype;
+    d.shape = SSAM_SHAPE_DENSE;
+    d.fw = fw;
+    d.fh = fh;
+    d.steps = 1;
+    d.boundary = SSAM_BOUNDARY_ZERO;
+    d.flags = 0;
+    d.coeffs = w.data();
+    return ssam_stencil2d(&d, src, dst, width, height, width_stride, nullptr, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssam_j2d5pt_f32(const float* src, float* dst, int width, int height, float fc, float fn,
+                    float fs, float fw, float fe, void* stream) {
+    const float c[5] = {fc, fn, fs, fw, fe};
+    return star_call<float>(SSAM_F32, src, dst, width, height, 1, c, 1, nullptr, stream);
+}
+int ssam_j2d5pt_f64(const double* src, double* dst, int width, int height, double fc, double fn,
+                    double fs, double fw, double fe, void* stream) {
+    const double c[5] = {fc, fn, fs, fw, fe};
+    return star_call<double>(SSAM_F64, src, dst, width, height, 1, c, 1, nullptr, stream);
+}
+int ssam_j2dstar_f32(const float* src, float* dst, int width, int height, int radius,
+                     const float* coefs, void* stream) {
+    return star_call<float>(SSAM_F32, src, dst, width, height, radius, coefs, 1, nullptr, stream);
+}
+int ssam_j2dstar_f64(const double* src, double* dst, int width, int height, int radius,
+                     const double* coefs, void* stream) {
+    return star_call<double>(SSAM_F64, src, dst, width, height, radius, coefs, 1, nullptr,
+                             stream);
+}
+int ssam_j2dbox_f32(const float* src, float* dst, int width, int height, int radius,
+                    const float* w, void* stream) {
+    return box_call<float>(SSAM_F32, src, dst, width, height, radius, w, stream);
+}
+int ssam_j2dbox_f64(const double* src, double* dst, int width, int height, int radius,
+                    const double* w, void* stream) {
+    return box_call<double>(SSAM_F64, src, dst, width, height, radius, w, stream);
+}
+int ssam_j2d5pt_temporal_f32(const float* src, float* dst, int width, int height, float fc,
+                             float fn, float fs, float fw, float fe, int steps, float* work,
+                             void* stream) {
+    const float c[5] = {fc, fn, fs, fw, fe};
+    return star_call<float>(SSAM_F32, src, dst, width, height, 1, c, steps, work, stream);
+}
+int ssam_j2d5pt_temporal_f64(const double* src, double* dst, int width, int height, double fc,
+                             double fn, double fs, double fw, double fe, int steps,
+                             double* work, void* stream) {
+    const double c[5] = {fc, fn, fs, fw, fe};
+    return star_call<double>(SSAM_F64, src, dst, width, height, 1, c, steps, work, stream);
+}
+int ssam_j2d9pt_temporal_f32(const float* src, float* dst, int width, int height, float fc,
+                             float fn0, float fs0, float fw0, float fe0, float fn1, float fs1,
+                             float fw1, float fe1, int steps, float* work, void* stream) {
+    const float c[9] = {fc, fn0, fs0, fw0, fe0, fn1, fs1, fw1, fe1};
+    return star_call<float>(SSAM_F32, src, dst, width, height, 2, c, steps, work, stream);
+}
+int ssam_j2d9pt_temporal_f64(const double* src, double* dst, int width, int height, double fc,
+                             double fn0, double fs0, double fw0, double fe0, double fn1,
+                             double fs1, double fw1, double fe1, int steps, double* work,
+                             void* stream) {
+    const double c[9] = {fc, fn0, fs0, fw0, fe0, fn1, fs1, fw1, fe1};
+    return star_call<double>(SSAM_F64, src, dst, width, height, 2, c, steps, work, stream);
+}
+int ssam_conv2d_f32(const float* src, float* dst, int width, int width_stride, int height,
+                    const float* weight, int fw, int fh, void* stream) {
+    return conv_call<float>(SSAM_F32, src, dst, width, width_stride, height, weight, fw, fh,
+                            stream);
+}
+int ssam_conv2d_f64(const double* src, double* dst, int width, int width_stride, int height,
+                    const double* weight, int fw, int fh, void* stream) {
+    return conv_call<double>(SSAM_F64, src, dst, width, width_stride, height, weight, fw, fh,
+                             stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ 3D
+namespace {
+
+// Reference 3D coefficients in the descriptor's tap order.
+// 7pt: tmp3d7pt-v2.cu:9-15 (z-1 .1, y-1 .2, x-1 .5, c .3, x+1 .6, y+1 .4, z+1 .7)
+const double kRef7[7] = {0.3, 0.5, 0.6, 0.2, 0.4, 0.1, 0.7};
+// 13pt: tmp3d13pt.cu:9-21
+const double kRef13[13] = {0.3, 1.1, 1.2, 0.6, 0.8, 0.2, 0.4, 1.0, 1.3, 0.7, 0.9, 0.1, 0.5};
+// poisson: tmpPoisson.cu:9-27 (centre 2.666, faces -0.166, (y,z) edges -0.0833,
+// (x,z) and (x,y) edges +0.0833)
+const double kRefPoisson[19] = {2.666,   -0.166,  -0.166,  -0.166,  -0.166,  -0.166,  -0.166,
+                                -0.0833, -0.0833, -0.0833, -0.0833, 0.0833,  0.0833,  0.0833,
+                                0.0833,  0.0833,  0.0833,  0.0833,  0.0833};
+// this library's Jacobi sweep for the 19-point Laplacian: faces 1/12, edges 1/24
+const double kJacobi19[19] = {0.0,        1.0 / 12,   1.0 / 12,   1.0 / 12,   1.0 / 12,
+                              1.0 / 12,   1.0 / 12,   1.0 / 24,   1.0 / 24,   1.0 / 24,
+                              1.0 / 24,   1.0 / 24,   1.0 / 24,   1.0 / 24,   1.0 / 24,
+                              1.0 / 24,   1.0 / 24,   1.0 / 24,   1.0 / 24};
+
+// Device scratch for residual partials: grown on demand, kept per device for the
+// life of the process (never a grid-sized buffer).
+struct Scratch {
+    std::mutex mu;
+    double* ptr[64] = {};
+    int cap[64] = {};
+};
+Scratch g_scratch;
+
+int get_scratch(int nwords, double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return SSAM_E_HIP;
+    std::lock_guard<std::mutex> lk(g_scratch.mu);
+    if (g_scratch.cap[dev] < nwords) {
+        if (g_scratch.ptr[dev]) (void)hipFree(g_scratch.ptr[dev]);
+        g_scratch.ptr[dev] = nullptr;
+        g_scratch.cap[dev] = 0;
+        void* p = nullptr;
+        if (hipMalloc(&p, static_cast<size_t>(nwords) * sizeof(double)) != hipSuccess)
+            return SSAM_E_HIP;
+        g_scratch.ptr[dev] = static_cast<double*>(p);
+        g_scratch.cap[dev] = nwords;
+    }
+    *out = g_scratch.ptr[dev];
+    return SSAM_OK;
+}
+
+int stencil3d_run(int storage, int shape, int radius, const double* taps, int steps,
+                  const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz, void* work,
+                  double* residual_l2, double res_scale, hipStream_t st) {
+    if (!src || !dst || !taps) return SSAM_E_INVALID;
+    if (storage != SSAM_F32 && storage != SSAM_F64) {
+        if (storage == SSAM_BF16 || storage == SSAM_F16) return SSAM_E_UNSUPPORTED;
+        return SSAM_E_INVALID;
+    }
+    if (nx < 1 || ny < 1 || nz < 1 || steps < 1) return SSAM_E_INVALID;
+    if (nx > (1 << 30) || ny > (1 << 30) || nz > (1 << 30)) return SSAM_E_UNSUPPORTED;
+    if (nx * ny > (1LL << 40) / nz) return SSAM_E_UNSUPPORTED;
+    if (shape == SSAM_SHAPE_STAR) {
+        if (radius < 1 || radius > 2) return SSAM_E_UNSUPPORTED;
+    } else if (shape == SSAM_SHAPE_FACE_EDGE) {
+        if (radius != 1) return SSAM_E_INVALID;
+    } else {
+        return SSAM_E_INVALID;
+    }
+    const int64_t bytes = nx * ny * nz * (storage == SSAM_F64 ? 8 : 4);
+    if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
+    if (steps > 1) {
+        if (!work) return SSAM_E_INVALID;
+        if (overlaps(work, bytes, dst, bytes) || overlaps(work, bytes, src, bytes))
+            return SSAM_E_INVALID;
+    }
+    Req3D rq;
+    rq.nx = nx;
+    rq.ny = ny;
+    rq.nz = nz;
+    rq.shape = shape == SSAM_SHAPE_FACE_EDGE ? 1 : 0;
+    rq.radius = radius;
+    rq.taps = taps;
+    rq.zseg = env_int("SSAM_ZSEG", 0);
+    rq.partials = nullptr;
+    rq.partials_cap = 0;
+    rq.res_out = nullptr;
+    rq.res_scale = res_scale;
+    double* scratch = nullptr;
+    int nparts = 0;
+    if (residual_l2) {
+        nparts = partials_needed(storage, rq);
+        const int rc = get_scratch(nparts + 1, &scratch);
+        if (rc != SSAM_OK) return rc;
+    }
+    const void* in = src;
+    for (int i = 0; i < steps; ++i) {
+        void* out = ((steps - 1 - i) % 2 == 0) ? dst : work;
+        rq.src = in;
+        rq.dst = out;
+        if (residual_l2 && i == steps - 1) {
+            rq.partials = scratch + 1;
+            rq.partials_cap = nparts;
+            rq.res_out = scratch;
+        }
+        const int rc = dispatch3d(storage, rq, st);
+        if (rc != SSAM_OK) return rc;
+        in = out;
+    }
+    if (residual_l2) {
+        double h = 0.0;
+        if (hipMemcpyAsync(&h, scratch, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return SSAM_E_HIP;
+        *residual_l2 = std::sqrt(h);
+    }
+    return SSAM_OK;
+}
+
+template <typename T>
+int ref3d(int dtype, int shape, int radius, const double* taps, const T* src, T* dst, int n,
+          int iters, T* work, void* stream) {
+    return stencil3d_run(dtype, shape, radius, taps, iters, src, dst, n, n, n, work, nullptr, 0.0,
+                         static_cast<hipStream_t>(stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssam_stencil3d(const ssam_stencil3d_desc* d, const void* src, void* dst, int64_t nx,
+                   int64_t ny, int64_t nz, void* work, void* stream) {
+    if (!d) return SSAM_E_INVALID;
+    if (d->boundary != SSAM_BOUNDARY_DIRICHLET0) return SSAM_E_UNSUPPORTED;
+    return stencil3d_run(d->storage, d->shape, d->radius, d->coeffs, d->steps, src, dst, nx, ny,
+                         nz, work, nullptr, 0.0, static_cast<hipStream_t>(stream));
+}
+
+int ssam_j3d7pt_f32(const float* src, float* dst, int n, int iters, float* work, void* stream) {
+    return ref3d<float>(SSAM_F32, SSAM_SHAPE_STAR, 1, kRef7, src, dst, n, iters, work, stream);
+}
+int ssam_j3d7pt_f64(const double* src, double* dst, int n, int iters, double* work, void* stream) {
+    return ref3d<double>(SSAM_F64, SSAM_SHAPE_STAR, 1, kRef7, src, dst, n, iters, work, stream);
+}
+int ssam_j3d13pt_f32(const float* src, float* dst, int n, int iters, float* work, void* stream) {
+    return ref3d<float>(SSAM_F32, SSAM_SHAPE_STAR, 2, kRef13, src, dst, n, iters, work, stream);
+}
+int ssam_j3d13pt_f64(const double* src, double* dst, int n, int iters, double* work,
+                     void* stream) {
+    return ref3d<double>(SSAM_F64, SSAM_SHAPE_STAR, 2, kRef13, src, dst, n, iters, work, stream);
+}
+int ssam_poisson_f32(const float* src, float* dst, int n, int iters, float* work, void* stream) {
+    return ref3d<float>(SSAM_F32, SSAM_SHAPE_FACE_EDGE, 1, kRefPoisson, src, dst, n, iters, work,
+                        stream);
+}
+int ssam_poisson_f64(const double* src, double* dst, int n, int iters, double* work,
+                     void* stream) {
+    return ref3d<double>(SSAM_F64, SSAM_SHAPE_FACE_EDGE, 1, kRefPoisson, src, dst, n, iters, work,
+                         stream);
+}
+
+int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz,
+                  int steps, void* work, double* residual_l2, void* stream) {
+    // r = (2F + E - 24u)/6 = 4 * (u' - u) with u' = (2F + E)/24
+    return stencil3d_run(storage, SSAM_SHAPE_FACE_EDGE, 1, kJacobi19, steps, src, dst, nx, ny, nz,
+                         work, residual_l2, 4.0, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

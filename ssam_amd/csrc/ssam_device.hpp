@@ -1,0 +1,117 @@
+// ssam_device.hpp -- device-side building blocks shared by the SSAM kernels
+// (gfx950, wave64).
+//
+//  * storage <-> accumulation conversion for F32 / F64 / BF16 / F16 grids;
+//  * whole-wave DPP lane shifts (wave_shr:1 / wave_shl:1), the systolic hop that
+//    replaces the reference's 32-lane __shfl_up/down_sync
+//    (ssai-2dconv/cudaLib.cuh:573-581);
+//  * V-wide per-lane row strip loads/stores (one dwordx4 per lane when the strip
+//    is 16 bytes and aligned), with the zero / clamp boundary rules.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ssam {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- storage types
+struct bf16_t { uint16_t bits; };
+struct f16_t { uint16_t bits; };
+
+template <typename S> struct Storage;
+template <> struct Storage<float> {
+    using acc = float;
+    __device__ static inline float load(float v) { return v; }
+    __device__ static inline float store(float v) { return v; }
+};
+template <> struct Storage<double> {
+    using acc = double;
+    __device__ static inline double load(double v) { return v; }
+    __device__ static inline double store(double v) { return v; }
+};
+template <> struct Storage<bf16_t> {
+    using acc = float;
+    __device__ static inline float load(bf16_t v) {
+        return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+    }
+    // round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+    __device__ static inline bf16_t store(float v) {
+        __bf16 b = static_cast<__bf16>(v);
+        bf16_t r;
+        __builtin_memcpy(&r.bits, &b, 2);
+        return r;
+    }
+};
+template <> struct Storage<f16_t> {
+    using acc = float;
+    __device__ static inline float load(f16_t v) {
+        _Float16 h;
+        __builtin_memcpy(&h, &v.bits, 2);
+        return static_cast<float>(h);
+    }
+    __device__ static inline f16_t store(float v) {
+        _Float16 h = static_cast<_Float16>(v);
+        f16_t r;
+        __builtin_memcpy(&r.bits, &h, 2);
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------- DPP lane shifts
+// DPP controls (GFX9 encoding): wave_shl:1 = 0x130 (lane i reads lane i+1),
+// wave_shr:1 = 0x138 (lane i reads lane i-1).  Lanes shifted in from outside the
+// wave read 0 (bound_ctrl); callers only consume them on halo lanes.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const uint64_t u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(u & 0xffffffffu), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                static_cast<uint32_t>(lo));
+}
+// value of lane i+1 (data moves toward lower lanes)
+template <typename A> __device__ __forceinline__ A from_right(A v) { return dpp_move<0x130>(v); }
+// value of lane i-1 (data moves toward higher lanes)
+template <typename A> __device__ __forceinline__ A from_left(A v) { return dpp_move<0x138>(v); }
+
+// ---------------------------------------------------------------- strip I/O
+// One lane's V consecutive elements of a row.  `full` = the whole strip lies
+// inside [0, width) and is aligned for a single vector access.
+template <typename S, int V> struct VecBits {
+    static constexpr int bytes = V * static_cast<int>(sizeof(S));
+    static_assert(bytes % 4 == 0, "strip must be a whole number of dwords");
+    using type = uint32_t __attribute__((ext_vector_type(bytes / 4)));
+};
+
+template <typename S, int V>
+__device__ __forceinline__ void load_strip_vec(const S* __restrict__ p, typename Storage<S>::acc (&out)[V]) {
+    using VB = typename VecBits<S, V>::type;
+    VB raw = *reinterpret_cast<const VB*>(p);
+    S tmp[V];
+    __builtin_memcpy(tmp, &raw, sizeof(tmp));
+#pragma unroll
+    for (int j = 0; j < V; ++j) out[j] = Storage<S>::load(tmp[j]);
+}
+
+template <typename S, int V>
+__device__ __forceinline__ void store_strip_vec(S* __restrict__ p, const typename Storage<S>::acc (&v)[V]) {
+    using VB = typename VecBits<S, V>::type;
+    S tmp[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) tmp[j] = Storage<S>::store(v[j]);
+    VB raw;
+    __builtin_memcpy(&raw, tmp, sizeof(tmp));
+    *reinterpret_cast<VB*>(p) = raw;
+}
+
+// Boundary rules for reads outside the grid.
+enum Boundary : int { kZero = 0, kClamp = 1, kHold = 2, kDirichlet0 = 3 };
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+}  // namespace ssam

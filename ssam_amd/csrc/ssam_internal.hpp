@@ -1,0 +1,44 @@
+// ssam_internal.hpp -- host-side request structs shared by the C-ABI layer
+// (api.cpp) and the per-storage-type dispatch units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/ssam.h"
+
+namespace ssam {
+
+// A validated 2D request: one launch of `steps` fused sweeps (steps in 1,2,4,8).
+struct Req2D {
+    const void* src;
+    void* dst;
+    int64_t width, height, pitch;
+    int shape;       // Shape2D (kStar / kBox; DENSE arrives embedded in a box)
+    int radius;      // filter half-width after embedding
+    int steps;       // fused steps of this launch
+    int boundary;    // Boundary
+    const double* taps;  // (2r+1)^2 row-major, host
+    int seg_rows;        // 0 = heuristic
+    int waves_per_simd;  // tile-count target for the heuristic
+};
+
+template <typename S> int dispatch2d(const Req2D& rq, hipStream_t st);
+
+// A validated 3D request: one sweep.
+struct Req3D {
+    const void* src;
+    void* dst;
+    int64_t nx, ny, nz;
+    int shape;           // Shape3D: 0 star, 1 face+edge (19 points)
+    int radius;
+    const double* taps;  // descriptor order (include/ssam.h)
+    int zseg;            // output planes per tile, 0 = heuristic
+    double* partials;    // device, per-tile residual partials (nullptr: none)
+    int partials_cap;
+    double* res_out;     // device, the summed residual^2
+    double res_scale;
+};
+
+int dispatch3d(int storage, const Req3D& rq, hipStream_t st);
+int partials_needed(int storage, const Req3D& rq);
+
+}  // namespace ssam

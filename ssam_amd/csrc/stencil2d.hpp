@@ -1,0 +1,390 @@
+// stencil2d.hpp -- SSAM register-cache systolic 2D stencil / correlation kernel
+// for gfx950 (wave64).
+//
+// Replaces the reference kernels
+//   stencil2d_v1::j2d5pt      (ssai-2dstencil/stencil2D_5pt-v1.cu:8-129)
+//   stencil2d_{9,13,17,21,25}pt::j2d*pt (stencil2D_*pt.cu, star radius 2..6)
+//   stencil2d_64pt/81/121pt::j2d*pt     (box radius 4 / 4 / 5)
+//   stencil2d_tmp::j2d5pt / j2d9pt      (stencil2d_temp_{5,9}pt.cu, fused steps)
+//   CONV2D::kernel_convolution2D        (ssai-2dconv/conv2D.cu:9-70)
+// with one template.  All compute out(x,y) = sum_{n,m} w[n][m] in(x+m-r, y+n-r),
+// i.e. correlation without flip, the same as the reference CPU gold
+// (ssai-2dconv/common.h:186-216).
+//
+// Execution model (MI355X-first, not a port of the 32-lane code):
+//  * One wave64 owns a column strip; lane l caches V consecutive columns
+//    (one dwordx4 for F32 V=4, F64 V=2, BF16/F16 V=8).  HL = ceil(T*r/V) lanes on
+//    each side are halo lanes: they load, compute and pass partial sums, but do
+//    not store.  Useful fraction (64-2*HL)/64 (reference: (32-2r)/32).
+//  * The wave streams down a segment of rows.  Each lane keeps a register ring of
+//    2r+1 (+prefetch) input rows; every tick loads one new row (prefetched PF rows
+//    ahead) and produces one output row per time step.  Each input element is
+//    read from HBM once per segment (plus 2*T*r halo rows per segment).
+//  * Horizontal taps: for output column j a lane forms the partial sum of its own
+//    columns (d = 0) and, for neighbours at lane distance d, the partial that the
+//    neighbour needs; partials travel lane to lane with DPP wave_shl:1 /
+//    wave_shr:1, fused into v_add_f32_dpp (the systolic hop; no LDS, no shuffle
+//    through memory).
+//  * T time steps run as a T-stage pipeline of register rings in one launch
+//    (temporal blocking); intermediate rows/columns outside the grid are forced to
+//    zero each step, so the fused result equals T zero-padded sweeps exactly as in
+//    the reference gold iteration (stencil2d_temp_5pt.cu:169-175).
+//  * Workgroups are remapped so each XCD gets a contiguous band of tiles
+//    (vertically adjacent segments share that XCD's L2 for their halo rows).
+#pragma once
+#include "ssam_device.hpp"
+
+namespace ssam {
+
+enum Shape2D : int { kStar = 0, kBox = 1 };
+
+template <typename A, int NT> struct Taps { A w[NT]; };
+
+struct Params2D {
+    int width, height;
+    long long pitch;  // elements between rows (src and dst)
+    int nstrips;      // column strips
+    int ntiles;       // nstrips * nsegs
+    int seg_rows;     // output rows per tile
+    int vec_ok;       // strips may use one vector access (pitch, pointers aligned)
+    int nblocks;      // gridDim.x (for the XCD remap)
+    int boundary;     // Boundary rule (kZero, kClamp, kHold)
+};
+
+constexpr int kWavesPerBlock2D = 4;
+
+// blockIdx -> tile-block so that XCD x (which receives blocks x, x+8, ...) works
+// on one contiguous range of tile-blocks.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+    const int n8 = (nblocks / 8) * 8;
+    if (b >= n8) return b;
+    return (b % 8) * (n8 / 8) + b / 8;
+}
+
+constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
+
+template <int R, int V, int T> struct Geom2D {
+    static constexpr int K = 2 * R + 1;
+    static constexpr int HL = (T * R + V - 1) / V;  // halo lanes per side
+    static constexpr int OUT_LANES = kWave - 2 * HL;
+    static constexpr int OW = OUT_LANES * V;        // output columns per strip
+    static_assert(OUT_LANES > 0, "radius*steps too large for one wave");
+};
+
+// ------------------------------------------------------------------ one output row
+// X[n][c]: rows y-r..y+r (n = 0..2r) of this lane's V columns.
+// out[j] = sum_n sum_m w[n][m+r] * in(col j+m, row n) across lanes.
+// w: taps (row-major (2r+1)^2) in the kernel-argument segment, read with scalar
+// loads at a uniform address.
+template <typename A, int R, int SHAPE, int V>
+__device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const A* __restrict__ w,
+                                            A (&out)[V]) {
+    constexpr int K = 2 * R + 1;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        // own columns (lane distance 0)
+        A acc = 0;
+#pragma unroll
+        for (int n = 0; n < K; ++n) {
+#pragma unroll
+            for (int m = -R; m <= R; ++m) {
+                if (SHAPE == kStar && n != R && m != 0) continue;
+                const int c = j + m;
+                if (c < 0 || c >= V) continue;
+                acc = __builtin_fma(w[n * K + m + R], X[n][c], acc);
+            }
+        }
+        // partials owed by lanes to the right (d > 0), chained systolically:
+        // a = T[DR]; a = shl(a) + T[d] ... ; acc += shl(a)
+        if (j + R >= V) {
+            const int DR = (j + R) / V;
+            A a = 0;
+#pragma unroll
+            for (int d = (R + V - 1) / V + 1; d >= 1; --d) {
+                if (d > DR) continue;
+                A t = 0;
+#pragma unroll
+                for (int n = 0; n < K; ++n) {
+#pragma unroll
+                    for (int m = -R; m <= R; ++m) {
+                        if (SHAPE == kStar && n != R && m != 0) continue;
+                        const int c = j + m;
+                        if (c < d * V || c >= (d + 1) * V) continue;
+                        t = __builtin_fma(w[n * K + m + R], X[n][c - d * V], t);
+                    }
+                }
+                a = (d == DR) ? t : from_right(a) + t;
+            }
+            acc += from_right(a);
+        }
+        // partials owed by lanes to the left (d < 0)
+        if (j - R < 0) {
+            const int DL = (R - j + V - 1) / V;
+            A b = 0;
+#pragma unroll
+            for (int e = (R + V - 1) / V + 1; e >= 1; --e) {
+                if (e > DL) continue;
+                const int d = -e;
+                A t = 0;
+#pragma unroll
+                for (int n = 0; n < K; ++n) {
+#pragma unroll
+                    for (int m = -R; m <= R; ++m) {
+                        if (SHAPE == kStar && n != R && m != 0) continue;
+                        const int c = j + m;
+                        if (c < d * V || c >= (d + 1) * V) continue;
+                        t = __builtin_fma(w[n * K + m + R], X[n][c - d * V], t);
+                    }
+                }
+                b = (e == DL) ? t : from_left(b) + t;
+            }
+            acc += from_left(b);
+        }
+        out[j] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ kernel
+// S storage type, R radius, SHAPE star/box, V columns per lane, T fused steps,
+// PF prefetch distance in rows, U ticks per unrolled block.
+//
+// Register rings (block-shift form): at the start of a block of U ticks the
+// input ring holds rows yc0-2R .. yc0+PF-1 in slots 0..K+PF-2; tick k loads row
+// yc0+k+PF into slot 2R+PF+k and reads rows yc0+k-2R .. yc0+k from slots
+// k .. k+2R.  Each intermediate step ring holds its last 2R rows in slots 0..2R-1
+// and tick k writes slot 2R+k.  After the block every ring shifts down by U
+// slots: (K+PF-1 + (T-1)*2R) * V register moves per U ticks.
+template <typename S, int R, int SHAPE, int V, int T, int PF, int U>
+struct Stencil2D {
+    using A = typename Storage<S>::acc;
+    using G = Geom2D<R, V, T>;
+    static constexpr int K = 2 * R + 1;
+    static constexpr int NT = K * K;
+    static constexpr int NR0 = K + PF + U - 1;  // input ring
+    static constexpr int NRS = K + U - 1;       // ring of each intermediate step
+    static constexpr int NSTAGE_RINGS = T > 1 ? T - 1 : 1;
+
+    struct Lane {
+        const S* __restrict__ src;
+        S* __restrict__ dst;
+        Params2D p;
+        int x0, ys, ye;
+        uint32_t xoff;  // byte offset of this lane's strip within a row (x0 >= 0)
+        bool lane_full, edge, store_lane;
+        bool inb[V];
+    };
+
+    // CHK = false: the wave's whole footprint (rows and columns, halo included) is
+    // inside the grid and vector-aligned, so rows load with one unconditional
+    // vector access and no boundary rule applies.
+    // Row base pointers are wave-uniform (SGPRs); a lane adds its 32-bit byte
+    // offset, so loads and stores use the SGPR-base + VGPR-offset address form.
+    __device__ static __forceinline__ const char* row_ptr(const Lane& L, int y) {
+        return reinterpret_cast<const char*>(L.src) +
+               static_cast<long long>(y) * L.p.pitch * static_cast<long long>(sizeof(S));
+    }
+    template <bool CHK>
+    __device__ static __forceinline__ void load_row(const Lane& L, int y, A (&dst)[V]) {
+        if constexpr (!CHK) {
+            load_strip_vec<S, V>(reinterpret_cast<const S*>(row_ptr(L, y) + L.xoff), dst);
+        } else {
+            int yy = y;
+            const bool clamp = L.p.boundary == kClamp;
+            if (clamp) {
+                yy = clampi(y, 0, L.p.height - 1);
+            } else if (y < 0 || y >= L.p.height) {
+#pragma unroll
+                for (int j = 0; j < V; ++j) dst[j] = 0;
+                return;
+            }
+            const char* row = row_ptr(L, yy);
+            if (L.lane_full && L.p.vec_ok) {
+                load_strip_vec<S, V>(reinterpret_cast<const S*>(row + L.xoff), dst);
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const int x = L.x0 + j;
+                    if (clamp) {
+                        const uint32_t o = static_cast<uint32_t>(clampi(x, 0, L.p.width - 1)) *
+                                           static_cast<uint32_t>(sizeof(S));
+                        dst[j] = Storage<S>::load(*reinterpret_cast<const S*>(row + o));
+                    } else {
+                        const uint32_t o = L.xoff + static_cast<uint32_t>(j * sizeof(S));
+                        dst[j] = L.inb[j] ? Storage<S>::load(*reinterpret_cast<const S*>(row + o))
+                                          : A(0);
+                    }
+                }
+            }
+        }
+    }
+
+    template <bool CHK>
+    __device__ static __forceinline__ void store_row(const Lane& L, int y, const A (&v)[V]) {
+        if (!L.store_lane) return;
+        char* row = reinterpret_cast<char*>(L.dst) +
+                    static_cast<long long>(y) * L.p.pitch * static_cast<long long>(sizeof(S));
+        if (!CHK || (L.lane_full && L.p.vec_ok)) {
+            store_strip_vec<S, V>(reinterpret_cast<S*>(row + L.xoff), v);
+        } else {
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+                if (L.inb[j])
+                    *reinterpret_cast<S*>(row + L.xoff + static_cast<uint32_t>(j * sizeof(S))) =
+                        Storage<S>::store(v[j]);
+        }
+    }
+
+    template <bool CHK, int KT>
+    __device__ static __forceinline__ void tick(const Lane& L, const A* __restrict__ w, int yc0,
+                                                A (&ring0)[NR0][V],
+                                                A (&rings)[NSTAGE_RINGS][NRS][V]) {
+        const int yc = yc0 + KT;
+        load_row<CHK>(L, yc + PF, ring0[2 * R + PF + KT]);
+        // taps come from the kernel-argument segment through scalar loads; the
+        // opaque zero offset keeps them from being hoisted out of the row loop
+        // into (uniform) VGPRs, which large boxes cannot afford
+        int zero = 0;
+        asm volatile("" : "+s"(zero));
+        const A* __restrict__ tw = w + zero;
+#pragma unroll
+        for (int t = 1; t <= T; ++t) {
+            A X[K][V];
+#pragma unroll
+            for (int n = 0; n < K; ++n)
+#pragma unroll
+                for (int j = 0; j < V; ++j)
+                    X[n][j] = (t == 1) ? ring0[KT + n][j] : rings[t - 2][KT + n][j];
+            A out[V];
+            stencil_row<A, R, SHAPE, V>(X, tw, out);
+            const int q = yc - t * R;  // grid row produced by this step
+            if (t < T) {
+                if constexpr (CHK) {
+                    // zero outside the grid so the next step sees zero padding
+                    if (q < 0 || q >= L.p.height) {
+#pragma unroll
+                        for (int j = 0; j < V; ++j) out[j] = 0;
+                    } else if (L.edge) {
+#pragma unroll
+                        for (int j = 0; j < V; ++j) out[j] = L.inb[j] ? out[j] : A(0);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < V; ++j) rings[t - 1][2 * R + KT][j] = out[j];
+            } else if (q >= L.ys && q < L.ye) {
+                if constexpr (CHK && T == 1) {
+                    if (L.p.boundary == kHold) {
+                        // border cells keep their input value
+                        const A* in = ring0[KT + R];
+                        const bool row_border = q < R || q >= L.p.height - R;
+#pragma unroll
+                        for (int j = 0; j < V; ++j) {
+                            const int x = L.x0 + j;
+                            const bool border = row_border || x < R || x >= L.p.width - R;
+                            out[j] = border ? in[j] : out[j];
+                        }
+                    }
+                }
+                store_row<CHK>(L, q, out);
+            }
+        }
+    }
+
+    template <bool CHK, int KT>
+    __device__ static __forceinline__ void block(const Lane& L, const A* __restrict__ w, int yc0,
+                                                 A (&ring0)[NR0][V],
+                                                 A (&rings)[NSTAGE_RINGS][NRS][V]) {
+        if constexpr (KT < U) {
+            tick<CHK, KT>(L, w, yc0, ring0, rings);
+            block<CHK, KT + 1>(L, w, yc0, ring0, rings);
+        }
+    }
+
+    template <bool CHK>
+    __device__ static __forceinline__ void stream(const Lane& L, const A* __restrict__ w, int c0,
+                                                  int nblk) {
+        A ring0[NR0][V];
+        A rings[NSTAGE_RINGS][NRS][V];
+#pragma unroll
+        for (int s = 0; s < NSTAGE_RINGS; ++s)
+#pragma unroll
+            for (int i = 0; i < NRS; ++i)
+#pragma unroll
+                for (int j = 0; j < V; ++j) rings[s][i][j] = 0;
+#pragma unroll
+        for (int i = 0; i < NR0; ++i)
+#pragma unroll
+            for (int j = 0; j < V; ++j) ring0[i][j] = 0;
+        // rows c0-2R .. c0+PF-1 into slots 0 .. K+PF-2
+#pragma unroll
+        for (int s = 0; s < K + PF - 1; ++s) load_row<CHK>(L, c0 - 2 * R + s, ring0[s]);
+        int yc0 = c0;
+        for (int b = 0; b < nblk; ++b, yc0 += U) {
+            block<CHK, 0>(L, w, yc0, ring0, rings);
+#pragma unroll
+            for (int i = 0; i < K + PF - 1; ++i)
+#pragma unroll
+                for (int j = 0; j < V; ++j) ring0[i][j] = ring0[i + U][j];
+#pragma unroll
+            for (int s = 0; s < NSTAGE_RINGS; ++s)
+#pragma unroll
+                for (int i = 0; i < 2 * R; ++i)
+#pragma unroll
+                    for (int j = 0; j < V; ++j) rings[s][i][j] = rings[s][i + U][j];
+        }
+    }
+
+    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const Params2D& p,
+                               const A* __restrict__ w) {
+        const int lane = threadIdx.x & (kWave - 1);
+        // wave index is uniform across the wave: keep the tile bookkeeping in SGPRs
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+        const int tile = xcd_remap(blockIdx.x, p.nblocks) * kWavesPerBlock2D + wv;
+        if (tile >= p.ntiles) return;  // whole wave leaves; no barriers are used
+        const int strip = tile % p.nstrips;
+        const int seg = tile / p.nstrips;
+
+        Lane L;
+        L.src = src;
+        L.dst = dst;
+        L.p = p;
+        const int strip_x = strip * G::OW;
+        L.x0 = strip_x + (lane - G::HL) * V;
+        L.xoff = static_cast<uint32_t>(L.x0) * static_cast<uint32_t>(sizeof(S));
+        L.ys = seg * p.seg_rows;
+        L.ye = min(p.height, L.ys + p.seg_rows);
+        L.lane_full = L.x0 >= 0 && L.x0 + V <= p.width;
+        L.edge = (strip_x - G::HL * V < 0) || (strip_x - G::HL * V + kWave * V > p.width);
+        L.store_lane = lane >= G::HL && lane < kWave - G::HL;
+#pragma unroll
+        for (int j = 0; j < V; ++j) L.inb[j] = (L.x0 + j >= 0) && (L.x0 + j < p.width);
+
+        // first tick consumes input row c0; step T first emits row ys at tick ys+T*R;
+        // the last tick emits row ye-1
+        const int c0 = L.ys - T * R + 2 * R;
+        const int ntick = (L.ye - 1 + T * R) - c0 + 1;
+        const int nblk = (ntick + U - 1) / U;
+        const int y_lo = c0 - 2 * R;
+        const int y_hi = c0 + nblk * U - 1 + PF;  // last row any tick loads
+        const bool interior = !L.edge && p.vec_ok && y_lo >= 0 && y_hi < p.height;
+        if (interior)
+            stream<false>(L, w, c0, nblk);
+        else
+            stream<true>(L, w, c0, nblk);
+    }
+};
+
+// The taps are the FIRST kernel argument, so they sit at offset 0 of the kernarg
+// segment; the kernel reads them there with scalar loads instead of copying the
+// by-value struct into private memory.
+template <typename S, int R, int SHAPE, int V, int T, int PF, int U, int WPE>
+__global__ __launch_bounds__(kWave * kWavesPerBlock2D) __attribute__((amdgpu_waves_per_eu(WPE)))
+void ssam2d_kernel(
+    Taps<typename Storage<S>::acc, (2 * R + 1) * (2 * R + 1)> tp, const S* __restrict__ src,
+    S* __restrict__ dst, Params2D p) {
+    (void)tp;
+    using A = typename Storage<S>::acc;
+    const A* __restrict__ w = (const A*)(__builtin_amdgcn_kernarg_segment_ptr());
+    Stencil2D<S, R, SHAPE, V, T, PF, U>::run(src, dst, p, w);
+}
+
+}  // namespace ssam

@@ -1,0 +1,130 @@
+// stencil2d_dispatch.hpp -- host-side launch of the 2D SSAM kernel for one storage
+// type: picks the compile-time instance (shape, radius, fused steps), sizes the
+// tiles and converts the taps to the accumulation type.
+#pragma once
+#include "stencil2d.hpp"
+#include "ssam_internal.hpp"
+
+namespace ssam {
+
+// Per-storage-type geometry: columns per lane (one 16-byte access) for a single
+// sweep, and for fused time steps (fewer columns keep the T-stage rings in VGPRs).
+template <typename S> struct Lanes2D;
+template <> struct Lanes2D<float> { static constexpr int V = 4, VT = 4; };
+template <> struct Lanes2D<double> { static constexpr int V = 2, VT = 2; };
+template <> struct Lanes2D<bf16_t> { static constexpr int V = 8, VT = 4; };
+template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VT = 4; };
+
+// Ticks per unrolled block (bounded code size: about 2048 FMAs per block for
+// stars, 1024 for boxes, whose instances are VALU-bound), prefetch distance in
+// rows, and the minimum waves per SIMD the register allocator must leave room for.
+template <int R, int SHAPE, int V, int T> struct Unroll2D {
+    static constexpr int TAPS = SHAPE == kStar ? 4 * R + 1 : (2 * R + 1) * (2 * R + 1);
+    static constexpr int BODY = TAPS * V * T;
+    static constexpr int U0 = (SHAPE == kStar ? 2048 : 1024) / BODY;
+    static constexpr int U = U0 < 1 ? 1 : (U0 > 4 ? 4 : U0);
+    static constexpr int PF = 3;
+    static constexpr int WPE = 2;
+};
+
+template <typename S, int R, int SHAPE, int V, int T>
+static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
+    using A = typename Storage<S>::acc;
+    using G = Geom2D<R, V, T>;
+    constexpr int K = 2 * R + 1;
+    constexpr int PF = Unroll2D<R, SHAPE, V, T>::PF;
+    constexpr int U = Unroll2D<R, SHAPE, V, T>::U;
+    constexpr int WPE = Unroll2D<R, SHAPE, V, T>::WPE;
+
+    Taps<A, K * K> tp;
+    for (int i = 0; i < K * K; ++i) tp.w[i] = static_cast<A>(rq.taps[i]);
+
+    Params2D p;
+    p.width = static_cast<int>(rq.width);
+    p.height = static_cast<int>(rq.height);
+    p.pitch = rq.pitch;
+    p.nstrips = static_cast<int>((rq.width + G::OW - 1) / G::OW);
+    p.boundary = rq.boundary;
+    const uintptr_t align = static_cast<uintptr_t>(V * sizeof(S) >= 16 ? 16 : V * sizeof(S));
+    p.vec_ok = (rq.pitch % V == 0) && (reinterpret_cast<uintptr_t>(rq.src) % align == 0) &&
+               (reinterpret_cast<uintptr_t>(rq.dst) % align == 0);
+    // rows per tile: enough tiles for ~target waves per SIMD, but at least a few
+    // multiples of the halo so the 2*T*r re-read rows stay a small fraction
+    int seg = rq.seg_rows;
+    if (seg <= 0) {
+        const long long target_waves = static_cast<long long>(rq.waves_per_simd) * 1024;
+        long long nsegs = (target_waves + p.nstrips - 1) / p.nstrips;
+        if (nsegs < 1) nsegs = 1;
+        seg = static_cast<int>((rq.height + nsegs - 1) / nsegs);
+        const int min_seg = 8 * T * R > 16 ? 8 * T * R : 16;
+        if (seg < min_seg) seg = min_seg;
+    }
+    if (seg > p.height) seg = p.height;
+    p.seg_rows = seg;
+    const int nsegs = (p.height + seg - 1) / seg;
+    const long long ntiles = static_cast<long long>(p.nstrips) * nsegs;
+    if (ntiles > (1LL << 30)) return SSAM_E_UNSUPPORTED;
+    p.ntiles = static_cast<int>(ntiles);
+    p.nblocks = (p.ntiles + kWavesPerBlock2D - 1) / kWavesPerBlock2D;
+
+    hipLaunchKernelGGL((ssam2d_kernel<S, R, SHAPE, V, T, PF, U, WPE>), dim3(p.nblocks),
+                       dim3(kWave * kWavesPerBlock2D), 0, stream, tp, static_cast<const S*>(rq.src),
+                       static_cast<S*>(rq.dst), p);
+    return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
+}
+
+// radius is a compile-time instance parameter; these expand the runtime switch
+template <typename S, int SHAPE, int V, int T>
+static int dispatch_radius(const Req2D& rq, hipStream_t st) {
+    switch (rq.radius) {
+#define SSAM_R(r) \
+    case r:       \
+        return launch2d_instance<S, r, SHAPE, V, T>(rq, st);
+        SSAM_R(1) SSAM_R(2) SSAM_R(3) SSAM_R(4) SSAM_R(5) SSAM_R(6)
+#undef SSAM_R
+        default:
+            break;
+    }
+    if constexpr (SHAPE == kBox && T == 1) {
+        switch (rq.radius) {
+#define SSAM_R(r) \
+    case r:       \
+        return launch2d_instance<S, r, SHAPE, V, T>(rq, st);
+            SSAM_R(7) SSAM_R(8) SSAM_R(9)
+#undef SSAM_R
+            default:
+                break;
+        }
+    }
+    return SSAM_E_UNSUPPORTED;
+}
+
+template <typename S>
+int dispatch2d(const Req2D& rq, hipStream_t st) {
+    constexpr int V = Lanes2D<S>::V, VT = Lanes2D<S>::VT;
+    if (rq.steps == 1) {
+        if (rq.shape == kStar) return dispatch_radius<S, kStar, V, 1>(rq, st);
+        return dispatch_radius<S, kBox, V, 1>(rq, st);
+    }
+    // fused time steps: star radius 1 and 2 (the reference's temporal 5-point and
+    // 9-point star kernels), T in {2, 4, 8}
+    if (rq.shape != kStar || rq.radius > 2) return SSAM_E_UNSUPPORTED;
+    switch (rq.steps) {
+        case 2:
+            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT, 2>(rq, st)
+                                  : launch2d_instance<S, 2, kStar, VT, 2>(rq, st);
+        case 4:
+            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT, 4>(rq, st)
+                                  : launch2d_instance<S, 2, kStar, VT, 4>(rq, st);
+        case 8:
+            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT, 8>(rq, st)
+                                  : launch2d_instance<S, 2, kStar, VT, 8>(rq, st);
+        default:
+            return SSAM_E_UNSUPPORTED;
+    }
+}
+
+}  // namespace ssam
+
+#define SSAM_INSTANTIATE_2D(S) \
+    template int ssam::dispatch2d<S>(const ssam::Req2D&, hipStream_t);

@@ -1,0 +1,125 @@
+"""C ABI checks that need no GPU: libssam.so loads, exports every function
+include/ssam.h declares, and rejects malformed requests with status codes before
+any HIP call (the reference harness has no such checks; its failure modes are
+asserts and exits, ssai-2dconv/cudaLib.cuh:42-51)."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+from ssam_amd import _lib as L
+
+FAKE_A = ctypes.c_void_p(0x10000000)
+FAKE_B = ctypes.c_void_p(0x20000000)
+FAKE_W = ctypes.c_void_p(0x30000000)
+
+
+def test_library_exports_every_declared_symbol():
+    syms = L.declared_symbols()
+    assert len(syms) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    lib = L.lib()
+    for s in syms:
+        assert hasattr(lib, s)
+
+
+def test_version_and_strerror():
+    lib = L.lib()
+    assert b"gfx950" in lib.ssam_version()
+    assert L.strerror(L.OK) == "success"
+    assert L.strerror(L.E_INVALID) == "invalid argument"
+    assert L.strerror(99) == "unknown status"
+
+
+def _desc(storage=L.F32, shape=L.STAR, k=3, steps=1, boundary=L.ZERO, coeffs=None):
+    c = np.ones((k, k)) if coeffs is None else coeffs
+    d = L.Stencil2DDesc()
+    d.storage, d.shape, d.fw, d.fh = storage, shape, c.shape[1], c.shape[0]
+    d.steps, d.boundary, d.flags = steps, boundary, 0
+    d.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    return d, c
+
+
+def _s2d(d, src=FAKE_A, dst=FAKE_B, w=64, h=64, pitch=64, work=None):
+    return L.lib().ssam_stencil2d(ctypes.byref(d) if d is not None else None, src, dst, w, h,
+                                  pitch, work, None)
+
+
+def test_stencil2d_rejects_malformed_requests():
+    d, c = _desc()
+    assert _s2d(None) == L.E_INVALID
+    assert _s2d(d, src=None) == L.E_INVALID
+    assert _s2d(d, dst=None) == L.E_INVALID
+    assert _s2d(d, w=0) == L.E_INVALID
+    assert _s2d(d, h=-1) == L.E_INVALID
+    assert _s2d(d, pitch=10) == L.E_INVALID                 # pitch < width
+    assert _s2d(d, dst=FAKE_A) == L.E_INVALID               # src == dst
+    assert _s2d(d, dst=ctypes.c_void_p(0x10000000 + 64)) == L.E_INVALID  # overlap
+    for bad in [dict(storage=7), dict(shape=9), dict(steps=0), dict(boundary=5)]:
+        dd, _ = _desc(**bad)
+        assert _s2d(dd) == L.E_INVALID, bad
+    dd, _ = _desc(k=4)                                      # even star
+    assert _s2d(dd) == L.E_INVALID
+    dd, _ = _desc(boundary=L.HOLD, steps=2)                 # HOLD is single-step
+    assert _s2d(dd) == L.E_INVALID
+    dd, _ = _desc(steps=3)                                  # 2+1 launches need work
+    assert _s2d(dd) == L.E_INVALID
+    dd, _ = _desc(steps=3)
+    assert _s2d(dd, work=FAKE_A) == L.E_INVALID             # work aliases src
+    dd, _ = _desc(shape=L.BOX, k=21)                        # radius 10 box
+    assert _s2d(dd) == L.E_UNSUPPORTED
+    dd, _ = _desc(shape=L.STAR, k=15)                       # radius 7 star
+    assert _s2d(dd) == L.E_UNSUPPORTED
+    dd, _ = _desc(shape=L.DENSE, coeffs=np.ones((3, 21)))
+    assert _s2d(dd) == L.E_UNSUPPORTED
+
+
+def test_reference_entries_validate():
+    lib = L.lib()
+    assert lib.ssam_j2dstar_f32(FAKE_A, FAKE_B, 64, 64, 1, None, None) == L.E_INVALID
+    c = (ctypes.c_float * 33)()
+    assert lib.ssam_j2dstar_f32(FAKE_A, FAKE_B, 64, 64, 8, c, None) == L.E_UNSUPPORTED
+    assert lib.ssam_j2dbox_f64(FAKE_A, FAKE_B, 64, 64, 10, c, None) == L.E_UNSUPPORTED
+    assert lib.ssam_j2d5pt_f32(FAKE_A, FAKE_A, 64, 64, 1, 1, 1, 1, 1, None) == L.E_INVALID
+    assert lib.ssam_conv2d_f32(FAKE_A, FAKE_B, 64, 64, 64, None, 3, 3, None) == L.E_INVALID
+    assert lib.ssam_conv2d_f32(FAKE_A, FAKE_B, 64, 64, 64, FAKE_W, 21, 3, None) == L.E_UNSUPPORTED
+
+
+def test_stencil3d_validates():
+    lib = L.lib()
+    c = np.ones(19)
+    d = L.Stencil3DDesc()
+    d.storage, d.shape, d.radius, d.steps, d.boundary, d.flags = L.F32, L.STAR, 1, 1, L.DIRICHLET0, 0
+    d.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    f = lambda **kw: lib.ssam_stencil3d(ctypes.byref(d), kw.get("src", FAKE_A),
+                                        kw.get("dst", FAKE_B), 16, 16, 16, kw.get("work"), None)
+    assert lib.ssam_stencil3d(None, FAKE_A, FAKE_B, 16, 16, 16, None, None) == L.E_INVALID
+    assert f(dst=FAKE_A) == L.E_INVALID
+    d.steps = 2
+    assert f() == L.E_INVALID                                # needs work
+    d.steps = 1
+    d.radius = 3
+    assert f() == L.E_UNSUPPORTED
+    d.radius = 1
+    d.storage = L.BF16
+    assert f() == L.E_UNSUPPORTED
+    d.storage = L.F32
+    d.boundary = L.ZERO
+    assert f() == L.E_UNSUPPORTED
+    assert lib.ssam_jacobi19(L.F32, FAKE_A, FAKE_B, 0, 4, 4, 1, None, None, None) == L.E_INVALID
+    assert lib.ssam_j3d7pt_f32(FAKE_A, FAKE_B, 16, 2, None, None) == L.E_INVALID
+
+
+def test_utilities_validate():
+    lib = L.lib()
+    assert lib.ssam_fill_uniform(None, L.F32, 10, 0, 0, 0.0, 1.0, None) == L.E_INVALID
+    assert lib.ssam_fill_uniform(FAKE_A, 9, 10, 0, 0, 0.0, 1.0, None) == L.E_INVALID
+    assert lib.ssam_fill_uniform(FAKE_A, L.F32, 0, 0, 0, 0.0, 1.0, None) == L.OK
+    assert lib.ssam_stream_copy(FAKE_A, FAKE_B, 10, None) == L.E_INVALID
+    assert lib.ssam_stream_copy(ctypes.c_void_p(0x10000004), FAKE_B, 16, None) == L.E_INVALID
+    assert lib.ssam_stream_copy(FAKE_A, FAKE_B, 0, None) == L.OK

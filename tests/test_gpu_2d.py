@@ -1,0 +1,241 @@
+"""2D parity on the GPU, through the C ABI (libssam.so).
+
+Bar: integer-valued data with integer taps (the reference harness's own inputs,
+stencil2D_*pt.cu `rand() % 256`) must match the reference gold bit for bit;
+floating-point results must satisfy |gpu - gold| <= tol * s with the per-point
+scale s = sum_i |w_i x_i| (gold of |w| over |x|), tol = 1e-5 for FP32, 1e-12 for
+FP64, and against the FP64 gold 1e-2 for BF16 storage, 1e-3 for FP16 storage.
+"""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle
+from ssam_amd import _lib as L
+from ssam_amd import api
+from ssam_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float32: 1e-5, np.float64: 1e-12}
+TDT = {np.float32: torch.float32, np.float64: torch.float64}
+
+
+def dev(x: np.ndarray, pitch_pad: int = 0, dtype=None) -> torch.Tensor:
+    t = torch.from_numpy(np.ascontiguousarray(x))
+    if dtype is not None:
+        t = t.to(dtype)
+    if pitch_pad:
+        h, w = x.shape
+        big = torch.zeros((h, w + pitch_pad), dtype=t.dtype)
+        big[:, :w] = t
+        return big.cuda()[:, :w]
+    return t.cuda()
+
+
+def run2d(x, filt, shape, steps=1, boundary=L.ZERO, pitch_pad=0, storage=None):
+    src = dev(x, pitch_pad, storage)
+    dst = torch.full_like(src, float("nan")) if pitch_pad == 0 else dev(np.full(x.shape, np.nan), pitch_pad, src.dtype)
+    work = torch.empty_like(dst) if steps > 1 else None
+    api.stencil2d(src, dst, filt, shape=shape, steps=steps, boundary=boundary, work=work)
+    torch.cuda.synchronize()
+    return dst.to(torch.float64 if storage is not None else dst.dtype).cpu().numpy()
+
+
+def scale_of(x, filt, steps=1):
+    s = np.abs(np.asarray(x, dtype=np.float64))
+    a = np.abs(np.asarray(filt, dtype=np.float64))
+    for _ in range(steps):
+        s = oracle.conv2d(s, a)
+    return s
+
+
+def assert_close(got, want, scale, tol, what=""):
+    e = oracle.err(got, want, scale)
+    assert np.isfinite(got).all(), f"{what}: non-finite output"
+    assert e["max_rel"] <= tol, f"{what}: {e}"
+
+
+# ------------------------------------------------------------------ reference KATs
+COEFFS = None
+
+
+def coeffs(golden_dir):
+    global COEFFS
+    if COEFFS is None:
+        COEFFS = json.loads((golden_dir / "coeffs.json").read_text())
+    return COEFFS
+
+
+@pytest.mark.parametrize("r", range(1, 7))
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_kat_star_integer_exact(golden_dir, r, dt):
+    d = np.load(golden_dir / f"kat_star_r{r}_int_{dt}.npz")
+    got = run2d(d["src"], d["filt"], L.STAR)
+    assert np.array_equal(got, d["out"])
+
+
+@pytest.mark.parametrize("k", [2, 3, 5, 7, 9, 15, 19])
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_kat_conv_integer_exact(golden_dir, k, dt):
+    d = np.load(golden_dir / f"kat_conv{k}_int_{dt}.npz")
+    got = run2d(d["src"], d["filt"], L.DENSE)
+    assert np.array_equal(got, d["out"])
+    # the reference-named entry: pitched rows, device weights (conv2D.cu:9-12)
+    src = dev(d["src"], pitch_pad=5)
+    dst = dev(np.zeros_like(d["src"]), pitch_pad=5)
+    api.conv2d(src, dst, torch.from_numpy(d["filt"]).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(dst.cpu().numpy(), d["out"])
+
+
+@pytest.mark.parametrize("name,r", [("box64", 4), ("box81", 4), ("box121", 5)])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_kat_box_reference_tables(golden_dir, name, r, dt):
+    d = np.load(golden_dir / f"kat_{name}_{np.dtype(dt).name}.npz")
+    got = run2d(d["src"], d["filt"], L.BOX)
+    assert_close(got, d["out"], scale_of(d["src"], d["filt"]), TOL[dt], name)
+    src, dst = dev(d["src"]), dev(np.zeros_like(d["src"]))
+    api.j2dbox(src, dst, r, d["filt"])
+    torch.cuda.synchronize()
+    assert_close(dst.cpu().numpy(), d["out"], scale_of(d["src"], d["filt"]), TOL[dt], name)
+
+
+@pytest.mark.parametrize("name,steps", [("temporal5", 5), ("temporal5", 8), ("temporal9", 3),
+                                        ("temporal9", 8)])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_kat_temporal(golden_dir, name, steps, dt):
+    d = np.load(golden_dir / f"kat_{name}_T{steps}_{np.dtype(dt).name}.npz")
+    got = run2d(d["src"], d["filt"], L.STAR, steps=steps)
+    sc = scale_of(d["src"], d["filt"], steps)
+    assert_close(got, d["out"], sc, TOL[dt] * steps, f"{name} T={steps}")
+    # reference-named temporal entries
+    f = d["filt"]
+    src, dst, work = dev(d["src"]), dev(np.zeros_like(d["src"])), dev(np.zeros_like(d["src"]))
+    if name == "temporal5":
+        api.j2d5pt_temporal(src, dst, f[1, 1], f[0, 1], f[2, 1], f[1, 0], f[1, 2], steps, work)
+    else:
+        c9 = [f[2, 2], f[1, 2], f[3, 2], f[2, 1], f[2, 3], f[0, 2], f[4, 2], f[2, 0], f[2, 4]]
+        api.j2d9pt_temporal(src, dst, c9, steps, work)
+    torch.cuda.synchronize()
+    assert_close(dst.cpu().numpy(), d["out"], sc, TOL[dt] * steps, f"{name} entry")
+
+
+def test_j2d5pt_and_star_entries(golden_dir):
+    d = np.load(golden_dir / "kat_star_r1_int_float32.npz")
+    f = d["filt"]
+    src, dst = dev(d["src"]), dev(np.zeros_like(d["src"]))
+    api.j2d5pt(src, dst, f[1, 1], f[0, 1], f[2, 1], f[1, 0], f[1, 2])
+    torch.cuda.synchronize()
+    assert np.array_equal(dst.cpu().numpy(), d["out"])
+    for r in range(2, 7):
+        d = np.load(golden_dir / f"kat_star_r{r}_int_float64.npz")
+        f = d["filt"]
+        c = [f[r, r]]
+        for k in range(r):
+            c += [f[r - 1 - k, r], f[r + 1 + k, r], f[r, r - 1 - k], f[r, r + 1 + k]]
+        src, dst = dev(d["src"]), dev(np.zeros_like(d["src"]))
+        api.j2dstar(src, dst, r, c)
+        torch.cuda.synchronize()
+        assert np.array_equal(dst.cpu().numpy(), d["out"]), r
+
+
+# ------------------------------------------------------------------ seeded parity
+SIZES = [(1, 1), (3, 7), (5, 64), (64, 5), (37, 251), (61, 248), (47, 249), (130, 497), (300, 520)]
+
+
+@pytest.mark.parametrize("shape,r", [(L.STAR, r) for r in range(1, 7)] + [(L.BOX, r) for r in range(1, 10)])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_parity_seeded_shapes(shape, r, dt):
+    k = 2 * r + 1
+    filt = wl.normalised_taps("star" if shape == L.STAR else "box", r, 100 + r)
+    for i, (h, w) in enumerate(SIZES):
+        x = oracle.uniform(1000 + i, h * w, -1, 1, dt).reshape(h, w)
+        f = filt.astype(dt)
+        got = run2d(x, f, shape, pitch_pad=(3 if i % 3 == 1 else 0))
+        want = oracle.conv2d(x, f)
+        assert_close(got, want, scale_of(x, f), TOL[dt], f"shape={shape} r={r} size={(h, w)}")
+
+
+@pytest.mark.parametrize("boundary", [L.CLAMP, L.HOLD])
+@pytest.mark.parametrize("shape,r", [(L.STAR, 1), (L.STAR, 3), (L.BOX, 2), (L.BOX, 5)])
+def test_parity_boundaries(boundary, shape, r):
+    dt = np.float64
+    f = wl.normalised_taps("star" if shape == L.STAR else "box", r, 7)
+    for (h, w) in [(29, 67), (130, 300), (4, 9)]:
+        x = oracle.uniform(h + w, h * w, 0, 1, dt).reshape(h, w)
+        got = run2d(x, f, shape, boundary=boundary)
+        if boundary == L.CLAMP:
+            want = oracle.conv2d(np.pad(x, r, mode="edge"), f)[r:-r, r:-r]
+            sc = scale_of(np.pad(x, r, mode="edge"), f)[r:-r, r:-r]
+        else:
+            want = oracle.conv2d(x, f)
+            want[:r, :], want[-r:, :], want[:, :r], want[:, -r:] = x[:r, :], x[-r:, :], x[:, :r], x[:, -r:]
+            sc = np.maximum(scale_of(x, f), np.abs(x))
+        assert_close(got, want, sc, 1e-12, f"boundary={boundary} r={r} {(h, w)}")
+
+
+@pytest.mark.parametrize("r", [1, 2])
+@pytest.mark.parametrize("steps", [1, 2, 3, 4, 5, 8, 11, 16])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_parity_temporal(r, steps, dt):
+    f = wl.normalised_taps("star", r, 31).astype(dt)
+    for (h, w) in [(45, 250), (200, 261), (17, 6)]:
+        x = oracle.uniform(steps + h, h * w, 0, 1, dt).reshape(h, w)
+        got = run2d(x, f, L.STAR, steps=steps)
+        want = oracle.conv2d_iter(x, f, steps)
+        assert_close(got, want, scale_of(x, f, steps), TOL[dt] * steps, f"T={steps} {(h, w)}")
+
+
+def test_box_multistep_chains_single_sweeps():
+    f = wl.normalised_taps("box", 1, 5)
+    x = oracle.uniform(9, 70 * 90, -1, 1, np.float64).reshape(70, 90)
+    got = run2d(x, f, L.BOX, steps=3)
+    want = oracle.conv2d_iter(x, f, 3)
+    assert_close(got, want, scale_of(x, f, 3), 1e-12, "box chained")
+
+
+@pytest.mark.parametrize("storage,tol", [(torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
+@pytest.mark.parametrize("shape,r,steps", [(L.STAR, 1, 1), (L.BOX, 2, 1), (L.BOX, 5, 1),
+                                           (L.STAR, 1, 8), (L.STAR, 2, 8), (L.STAR, 4, 1)])
+def test_storage_precision_vs_fp64(storage, tol, shape, r, steps):
+    f = wl.normalised_taps("star" if shape == L.STAR else "box", r, 77)
+    h, w = 150, 700
+    x = oracle.uniform(5, h * w, -1, 1, np.float64).reshape(h, w)
+    got = run2d(x, f, shape, steps=steps, storage=storage)
+    want = oracle.conv2d_iter(x, f, steps)
+    assert_close(got, want, scale_of(x, f, steps), tol, f"{storage} r={r} T={steps}")
+
+
+# ------------------------------------------------------------------ full size
+def _band_check(x, got, f, rows, dt, steps=1):
+    r = f.shape[0] // 2 * steps
+    h = x.shape[0]
+    for y0 in rows:
+        lo, hi = max(0, y0 - r), min(h, y0 + 16 + r)
+        slab = x[lo:hi]
+        want = oracle.conv2d_iter(slab, f, steps) if steps > 1 else oracle.conv2d(slab, f)
+        sc = scale_of(slab, f, steps)
+        a, b = y0 - lo, y0 - lo + min(16, h - y0)
+        # rows away from the slab's artificial top/bottom edge are exact gold rows
+        if lo == 0 or a >= r:
+            assert_close(got[y0:y0 + (b - a)], want[a:b], sc[a:b], TOL[dt] * steps, f"band {y0}")
+
+
+@pytest.mark.parametrize("shape,r,steps", [(L.STAR, 1, 1), (L.BOX, 5, 1), (L.STAR, 1, 8)])
+def test_full_size_8192_bands(shape, r, steps):
+    dt = np.float32
+    n = 8192
+    src = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    api.fill_uniform(src, 0, -1.0, 1.0)
+    f = wl.normalised_taps("star" if shape == L.STAR else "box", r, 3).astype(dt)
+    dst = torch.empty_like(src)
+    work = torch.empty_like(src) if steps > 8 else None
+    api.stencil2d(src, dst, f, shape=shape, steps=steps, work=work)
+    torch.cuda.synchronize()
+    x = src.cpu().numpy()
+    got = dst.cpu().numpy()
+    assert np.isfinite(got).all()
+    _band_check(x, got, f, [0, 2040, 4096 - 7, 8192 - 16], dt, steps)
