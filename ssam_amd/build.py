@@ -78,12 +78,37 @@ def build(jobs: int | None = None, verbose: bool = False) -> pathlib.Path:
     return LIB
 
 
+DRIVERS = {
+    "ssai-2dstencil": "ssai_2dstencil.cpp",
+    "ssai-3dstencil": "ssai_3dstencil.cpp",
+    "ssai-2dconv": "ssai_2dconv.cpp",
+}
+BIN = HERE / "bin"
+
+
+def build_drivers() -> list[pathlib.Path]:
+    """The reference-command-line drivers, linked against libssam.so."""
+    BIN.mkdir(exist_ok=True)
+    out = []
+    for exe, src in DRIVERS.items():
+        target = BIN / exe
+        cmd = [HIPCC, "-O3", "-std=c++17", "-fopenmp", str(HERE / "drivers" / src), "-o",
+               str(target), f"-L{HERE}", "-lssam", "-Wl,-rpath,$ORIGIN/.."]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode:
+            raise RuntimeError(f"driver {exe} failed:\n{p.stdout}{p.stderr}")
+        out.append(target)
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("-v", action="store_true")
     a = ap.parse_args()
     print(build(a.j, a.v))
+    for p in build_drivers():
+        print(p)
 
 
 if __name__ == "__main__":

@@ -52,9 +52,11 @@ int launch2d(int storage, const Req2D& rq, hipStream_t st) {
     }
 }
 
-// Largest fused step count the kernels provide for this shape.
-int max_fused(int shape, int radius) {
-    return (shape == SSAM_SHAPE_STAR && radius <= 2) ? kMaxFusedSteps : 1;
+// Largest fused step count the kernels provide for this shape and storage type.
+int max_fused(int storage, int shape, int radius) {
+    if (shape != SSAM_SHAPE_STAR || radius > 2) return 1;
+    if (radius == 2 && storage == SSAM_F64) return 4;
+    return kMaxFusedSteps;
 }
 
 }  // namespace
@@ -112,7 +114,7 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
     if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
 
     // split the steps into fused launches (8, 4, 2, 1) chained through work/dst
-    const int fmax = max_fused(shape, R);
+    const int fmax = max_fused(d->storage, shape, R);
     std::vector<int> chunks;
     for (int left = d->steps; left > 0;) {
         int t = fmax;

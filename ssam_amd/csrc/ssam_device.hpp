@@ -58,6 +58,11 @@ template <> struct Storage<f16_t> {
     }
 };
 
+// ---------------------------------------------------------------- fused multiply-add
+// in the accumulation type (v_fmac_f32 / v_fma_f64); never promotes FP32 to FP64
+__device__ __forceinline__ float fmaa(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaa(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // ---------------------------------------------------------------- DPP lane shifts
 // DPP controls (GFX9 encoding): wave_shl:1 = 0x130 (lane i reads lane i+1),
 // wave_shr:1 = 0x138 (lane i reads lane i-1).  Lanes shifted in from outside the

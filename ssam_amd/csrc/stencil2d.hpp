@@ -74,70 +74,55 @@ template <int R, int V, int T> struct Geom2D {
 // ------------------------------------------------------------------ one output row
 // X[n][c]: rows y-r..y+r (n = 0..2r) of this lane's V columns.
 // out[j] = sum_n sum_m w[n][m+r] * in(col j+m, row n) across lanes.
-// w: taps (row-major (2r+1)^2) in the kernel-argument segment, read with scalar
-// loads at a uniform address.
+//
+// Every tap (n, m) is read once (scalar load, uniform) and feeds its V FMAs at
+// once, into the partial sum P[j][d] that belongs to lane distance
+// d = floor((j+m)/V): d = 0 is this lane's own output, d > 0 is owed by the lane d
+// to the right, d < 0 by the lane -d to the left.  Taps therefore stay live only
+// briefly (large boxes do not run out of scalar registers).  The partials then
+// travel systolically: a = P[DR]; a = shl(a) + P[DR-1]; ...; out += shl(a)
+// (wave_shl:1, fused into v_add_f32_dpp), and the mirror image with wave_shr:1.
 template <typename A, int R, int SHAPE, int V>
 __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const A* __restrict__ w,
                                             A (&out)[V]) {
     constexpr int K = 2 * R + 1;
+    constexpr int D = (R + V - 1) / V;  // largest lane distance
+    A P[V][2 * D + 1];
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+#pragma unroll
+        for (int d = 0; d < 2 * D + 1; ++d) P[j][d] = 0;
+#pragma unroll
+    for (int n = 0; n < K; ++n) {
+#pragma unroll
+        for (int m = -R; m <= R; ++m) {
+            if (SHAPE == kStar && n != R && m != 0) continue;
+            const A wt = w[n * K + m + R];
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int c = j + m;
+                const int d = c >= 0 ? c / V : -((-c + V - 1) / V);
+                P[j][d + D] = fmaa(wt, X[n][c - d * V], P[j][d + D]);
+            }
+        }
+    }
 #pragma unroll
     for (int j = 0; j < V; ++j) {
-        // own columns (lane distance 0)
-        A acc = 0;
+        A acc = P[j][D];
+        const int DR = (j + R) / V;               // farthest right lane with a partial
+        const int DL = R > j ? (R - j + V - 1) / V : 0;
+        if (DR >= 1) {
+            A a = P[j][D + DR];
 #pragma unroll
-        for (int n = 0; n < K; ++n) {
-#pragma unroll
-            for (int m = -R; m <= R; ++m) {
-                if (SHAPE == kStar && n != R && m != 0) continue;
-                const int c = j + m;
-                if (c < 0 || c >= V) continue;
-                acc = __builtin_fma(w[n * K + m + R], X[n][c], acc);
-            }
-        }
-        // partials owed by lanes to the right (d > 0), chained systolically:
-        // a = T[DR]; a = shl(a) + T[d] ... ; acc += shl(a)
-        if (j + R >= V) {
-            const int DR = (j + R) / V;
-            A a = 0;
-#pragma unroll
-            for (int d = (R + V - 1) / V + 1; d >= 1; --d) {
-                if (d > DR) continue;
-                A t = 0;
-#pragma unroll
-                for (int n = 0; n < K; ++n) {
-#pragma unroll
-                    for (int m = -R; m <= R; ++m) {
-                        if (SHAPE == kStar && n != R && m != 0) continue;
-                        const int c = j + m;
-                        if (c < d * V || c >= (d + 1) * V) continue;
-                        t = __builtin_fma(w[n * K + m + R], X[n][c - d * V], t);
-                    }
-                }
-                a = (d == DR) ? t : from_right(a) + t;
-            }
+            for (int d = D; d >= 1; --d)
+                if (d < DR) a = from_right(a) + P[j][D + d];
             acc += from_right(a);
         }
-        // partials owed by lanes to the left (d < 0)
-        if (j - R < 0) {
-            const int DL = (R - j + V - 1) / V;
-            A b = 0;
+        if (DL >= 1) {
+            A b = P[j][D - DL];
 #pragma unroll
-            for (int e = (R + V - 1) / V + 1; e >= 1; --e) {
-                if (e > DL) continue;
-                const int d = -e;
-                A t = 0;
-#pragma unroll
-                for (int n = 0; n < K; ++n) {
-#pragma unroll
-                    for (int m = -R; m <= R; ++m) {
-                        if (SHAPE == kStar && n != R && m != 0) continue;
-                        const int c = j + m;
-                        if (c < d * V || c >= (d + 1) * V) continue;
-                        t = __builtin_fma(w[n * K + m + R], X[n][c - d * V], t);
-                    }
-                }
-                b = (e == DL) ? t : from_left(b) + t;
-            }
+            for (int e = D; e >= 1; --e)
+                if (e < DL) b = from_left(b) + P[j][D - e];
             acc += from_left(b);
         }
         out[j] = acc;
@@ -146,23 +131,32 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const A*
 
 // ------------------------------------------------------------------ kernel
 // S storage type, R radius, SHAPE star/box, V columns per lane, T fused steps,
-// PF prefetch distance in rows, U ticks per unrolled block.
+// PF prefetch distance in rows, RING ring-slot form, U ticks per unrolled block.
 //
-// Register rings (block-shift form): at the start of a block of U ticks the
-// input ring holds rows yc0-2R .. yc0+PF-1 in slots 0..K+PF-2; tick k loads row
-// yc0+k+PF into slot 2R+PF+k and reads rows yc0+k-2R .. yc0+k from slots
-// k .. k+2R.  Each intermediate step ring holds its last 2R rows in slots 0..2R-1
-// and tick k writes slot 2R+k.  After the block every ring shifts down by U
-// slots: (K+PF-1 + (T-1)*2R) * V register moves per U ticks.
-template <typename S, int R, int SHAPE, int V, int T, int PF, int U>
+// Register rings.  Input row y lives in ring slot (y - (c0-2R)) mod NR0 where c0
+// is the first consumed row; the rows of an intermediate step t produced at tick
+// k live in slot (k + 2R) mod NRS.  Tick k loads row c0+k+PF and reads rows
+// c0+k-2R .. c0+k (slots k .. k+2R).
+//  * RING: the loop body is one full ring period (U = lcm(NR0, NRS) ticks), so
+//    every slot index is a compile-time register and nothing ever moves; loads
+//    stay in flight across ticks and blocks.
+//  * !RING (large radius): U ticks per block; the block's U row loads are issued
+//    at its start, and after the block the rings shift down by U slots with
+//    register moves, which only touch rows whose loads had the whole block to
+//    land.  Used where a full ring period would not fit the instruction cache.
+template <typename S, int R, int SHAPE, int V, int T, int PF, bool RING, int U>
 struct Stencil2D {
     using A = typename Storage<S>::acc;
     using G = Geom2D<R, V, T>;
     static constexpr int K = 2 * R + 1;
     static constexpr int NT = K * K;
-    static constexpr int NR0 = K + PF + U - 1;  // input ring
-    static constexpr int NRS = K + U - 1;       // ring of each intermediate step
+    static constexpr int NR0 = RING ? K + PF : K + PF + U - 1;  // input ring
+    static constexpr int NRS = RING ? K : K + U - 1;            // ring of each step
     static constexpr int NSTAGE_RINGS = T > 1 ? T - 1 : 1;
+    static_assert(!RING || (U % NR0 == 0 && (T == 1 || U % NRS == 0)), "ring period");
+
+    static constexpr int s0(int k, int n) { return RING ? (k + n) % NR0 : k + n; }
+    static constexpr int sS(int k, int n) { return RING ? (k + n) % NRS : k + n; }
 
     struct Lane {
         const S* __restrict__ src;
@@ -239,7 +233,7 @@ struct Stencil2D {
                                                 A (&ring0)[NR0][V],
                                                 A (&rings)[NSTAGE_RINGS][NRS][V]) {
         const int yc = yc0 + KT;
-        load_row<CHK>(L, yc + PF, ring0[2 * R + PF + KT]);
+        if constexpr (RING) load_row<CHK>(L, yc + PF, ring0[s0(KT, PF + 2 * R)]);
         // taps come from the kernel-argument segment through scalar loads; the
         // opaque zero offset keeps them from being hoisted out of the row loop
         // into (uniform) VGPRs, which large boxes cannot afford
@@ -253,7 +247,7 @@ struct Stencil2D {
             for (int n = 0; n < K; ++n)
 #pragma unroll
                 for (int j = 0; j < V; ++j)
-                    X[n][j] = (t == 1) ? ring0[KT + n][j] : rings[t - 2][KT + n][j];
+                    X[n][j] = (t == 1) ? ring0[s0(KT, n)][j] : rings[t - 2][sS(KT, n)][j];
             A out[V];
             stencil_row<A, R, SHAPE, V>(X, tw, out);
             const int q = yc - t * R;  // grid row produced by this step
@@ -269,12 +263,12 @@ struct Stencil2D {
                     }
                 }
 #pragma unroll
-                for (int j = 0; j < V; ++j) rings[t - 1][2 * R + KT][j] = out[j];
+                for (int j = 0; j < V; ++j) rings[t - 1][sS(KT, 2 * R)][j] = out[j];
             } else if (q >= L.ys && q < L.ye) {
                 if constexpr (CHK && T == 1) {
                     if (L.p.boundary == kHold) {
                         // border cells keep their input value
-                        const A* in = ring0[KT + R];
+                        const A* in = ring0[s0(KT, R)];
                         const bool row_border = q < R || q >= L.p.height - R;
 #pragma unroll
                         for (int j = 0; j < V; ++j) {
@@ -314,22 +308,28 @@ struct Stencil2D {
         for (int i = 0; i < NR0; ++i)
 #pragma unroll
             for (int j = 0; j < V; ++j) ring0[i][j] = 0;
-        // rows c0-2R .. c0+PF-1 into slots 0 .. K+PF-2
+        // rows c0-2R .. c0+PF-1 into slots 0 .. 2R+PF-1
 #pragma unroll
-        for (int s = 0; s < K + PF - 1; ++s) load_row<CHK>(L, c0 - 2 * R + s, ring0[s]);
+        for (int s = 0; s < 2 * R + PF; ++s) load_row<CHK>(L, c0 - 2 * R + s, ring0[s]);
         int yc0 = c0;
         for (int b = 0; b < nblk; ++b, yc0 += U) {
+            if constexpr (!RING) {
+#pragma unroll
+                for (int k = 0; k < U; ++k) load_row<CHK>(L, yc0 + k + PF, ring0[2 * R + PF + k]);
+            }
             block<CHK, 0>(L, w, yc0, ring0, rings);
+            if constexpr (!RING) {
 #pragma unroll
-            for (int i = 0; i < K + PF - 1; ++i)
+                for (int i = 0; i < 2 * R + PF; ++i)
 #pragma unroll
-                for (int j = 0; j < V; ++j) ring0[i][j] = ring0[i + U][j];
+                    for (int j = 0; j < V; ++j) ring0[i][j] = ring0[i + U][j];
 #pragma unroll
-            for (int s = 0; s < NSTAGE_RINGS; ++s)
+                for (int s = 0; s < NSTAGE_RINGS; ++s)
 #pragma unroll
-                for (int i = 0; i < 2 * R; ++i)
+                    for (int i = 0; i < 2 * R; ++i)
 #pragma unroll
-                    for (int j = 0; j < V; ++j) rings[s][i][j] = rings[s][i + U][j];
+                        for (int j = 0; j < V; ++j) rings[s][i][j] = rings[s][i + U][j];
+            }
         }
     }
 
@@ -376,7 +376,7 @@ struct Stencil2D {
 // The taps are the FIRST kernel argument, so they sit at offset 0 of the kernarg
 // segment; the kernel reads them there with scalar loads instead of copying the
 // by-value struct into private memory.
-template <typename S, int R, int SHAPE, int V, int T, int PF, int U, int WPE>
+template <typename S, int R, int SHAPE, int V, int T, int PF, bool RING, int U, int WPE>
 __global__ __launch_bounds__(kWave * kWavesPerBlock2D) __attribute__((amdgpu_waves_per_eu(WPE)))
 void ssam2d_kernel(
     Taps<typename Storage<S>::acc, (2 * R + 1) * (2 * R + 1)> tp, const S* __restrict__ src,
@@ -384,7 +384,7 @@ void ssam2d_kernel(
     (void)tp;
     using A = typename Storage<S>::acc;
     const A* __restrict__ w = (const A*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Stencil2D<S, R, SHAPE, V, T, PF, U>::run(src, dst, p, w);
+    Stencil2D<S, R, SHAPE, V, T, PF, RING, U>::run(src, dst, p, w);
 }
 
 }  // namespace ssam

@@ -8,22 +8,32 @@
 namespace ssam {
 
 // Per-storage-type geometry: columns per lane (one 16-byte access) for a single
-// sweep, and for fused time steps (fewer columns keep the T-stage rings in VGPRs).
+// sweep, and for fused time steps of radius 1 / radius 2 (fewer columns keep the
+// T-stage register rings in VGPRs), and the most steps fused per launch at
+// radius 2.
 template <typename S> struct Lanes2D;
-template <> struct Lanes2D<float> { static constexpr int V = 4, VT = 4; };
-template <> struct Lanes2D<double> { static constexpr int V = 2, VT = 2; };
-template <> struct Lanes2D<bf16_t> { static constexpr int V = 8, VT = 4; };
-template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VT = 4; };
+template <> struct Lanes2D<float> { static constexpr int V = 4, VT1 = 4, VT2 = 2, TMAX2 = 8; };
+template <> struct Lanes2D<double> { static constexpr int V = 2, VT1 = 2, VT2 = 2, TMAX2 = 4; };
+template <> struct Lanes2D<bf16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
+template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
 
-// Ticks per unrolled block (bounded code size: about 2048 FMAs per block for
-// stars, 1024 for boxes, whose instances are VALU-bound), prefetch distance in
-// rows, and the minimum waves per SIMD the register allocator must leave room for.
+
+// Loop form per instance.  RING (no register moves): the body is a full ring
+// period, lcm(2r+1+PF, 2r+1) ticks, used while that stays under about 4096 FMAs
+// of code.  Otherwise blocks of U ticks with a shift (U bounded by about 1024
+// FMAs per block).  Also the minimum waves per SIMD the register allocator must
+// leave room for.
 template <int R, int SHAPE, int V, int T> struct Unroll2D {
-    static constexpr int TAPS = SHAPE == kStar ? 4 * R + 1 : (2 * R + 1) * (2 * R + 1);
+    static constexpr int K = 2 * R + 1;
+    static constexpr int TAPS = SHAPE == kStar ? 4 * R + 1 : K * K;
     static constexpr int BODY = TAPS * V * T;
-    static constexpr int U0 = (SHAPE == kStar ? 2048 : 1024) / BODY;
-    static constexpr int U = U0 < 1 ? 1 : (U0 > 4 ? 4 : U0);
-    static constexpr int PF = 3;
+    static constexpr int PF_RING = T > 1 ? K : (R <= 2 ? 5 : 3);
+    static constexpr int NR0 = K + PF_RING;
+    static constexpr int PERIOD = T > 1 ? (NR0 / cgcd(NR0, K)) * K : NR0;
+    static constexpr bool RING = PERIOD * BODY <= 2048;
+    static constexpr int U0 = 1024 / BODY;
+    static constexpr int U = RING ? PERIOD : (U0 < 1 ? 1 : (U0 > 4 ? 4 : U0));
+    static constexpr int PF = RING ? PF_RING : (U > 3 ? U : 3);
     static constexpr int WPE = 2;
 };
 
@@ -32,9 +42,11 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     using A = typename Storage<S>::acc;
     using G = Geom2D<R, V, T>;
     constexpr int K = 2 * R + 1;
-    constexpr int PF = Unroll2D<R, SHAPE, V, T>::PF;
-    constexpr int U = Unroll2D<R, SHAPE, V, T>::U;
-    constexpr int WPE = Unroll2D<R, SHAPE, V, T>::WPE;
+    using UR = Unroll2D<R, SHAPE, V, T>;
+    constexpr int PF = UR::PF;
+    constexpr bool RING = UR::RING;
+    constexpr int U = UR::U;
+    constexpr int WPE = UR::WPE;
 
     Taps<A, K * K> tp;
     for (int i = 0; i < K * K; ++i) tp.w[i] = static_cast<A>(rq.taps[i]);
@@ -67,7 +79,7 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     p.ntiles = static_cast<int>(ntiles);
     p.nblocks = (p.ntiles + kWavesPerBlock2D - 1) / kWavesPerBlock2D;
 
-    hipLaunchKernelGGL((ssam2d_kernel<S, R, SHAPE, V, T, PF, U, WPE>), dim3(p.nblocks),
+    hipLaunchKernelGGL((ssam2d_kernel<S, R, SHAPE, V, T, PF, RING, U, WPE>), dim3(p.nblocks),
                        dim3(kWave * kWavesPerBlock2D), 0, stream, tp, static_cast<const S*>(rq.src),
                        static_cast<S*>(rq.dst), p);
     return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
@@ -101,7 +113,7 @@ static int dispatch_radius(const Req2D& rq, hipStream_t st) {
 
 template <typename S>
 int dispatch2d(const Req2D& rq, hipStream_t st) {
-    constexpr int V = Lanes2D<S>::V, VT = Lanes2D<S>::VT;
+    constexpr int V = Lanes2D<S>::V, VT1 = Lanes2D<S>::VT1, VT2 = Lanes2D<S>::VT2;
     if (rq.steps == 1) {
         if (rq.shape == kStar) return dispatch_radius<S, kStar, V, 1>(rq, st);
         return dispatch_radius<S, kBox, V, 1>(rq, st);
@@ -111,14 +123,15 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
     if (rq.shape != kStar || rq.radius > 2) return SSAM_E_UNSUPPORTED;
     switch (rq.steps) {
         case 2:
-            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT, 2>(rq, st)
-                                  : launch2d_instance<S, 2, kStar, VT, 2>(rq, st);
+            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT1, 2>(rq, st)
+                                  : launch2d_instance<S, 2, kStar, VT2, 2>(rq, st);
         case 4:
-            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT, 4>(rq, st)
-                                  : launch2d_instance<S, 2, kStar, VT, 4>(rq, st);
+            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT1, 4>(rq, st)
+                                  : launch2d_instance<S, 2, kStar, VT2, 4>(rq, st);
         case 8:
-            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT, 8>(rq, st)
-                                  : launch2d_instance<S, 2, kStar, VT, 8>(rq, st);
+            if (rq.radius == 1) return launch2d_instance<S, 1, kStar, VT1, 8>(rq, st);
+            if constexpr (Lanes2D<S>::TMAX2 >= 8) return launch2d_instance<S, 2, kStar, VT2, 8>(rq, st);
+            return SSAM_E_UNSUPPORTED;
         default:
             return SSAM_E_UNSUPPORTED;
     }

@@ -81,7 +81,8 @@ struct Stencil3D {
         S* dst;
         Params3D p;
         int lane, x0, y0;
-        uint32_t xoff, hoff;
+        uint32_t xoff;
+        int hx;  // first column of this lane's halo load (always in [0, nx))
         bool full;
     };
 
@@ -103,9 +104,10 @@ struct Stencil3D {
                 }
             }
 #pragma unroll
-            for (int k = 0; k < R; ++k)
-                P.halo[i][k] = Storage<S>::load(
-                    *reinterpret_cast<const S*>(row + c.hoff + static_cast<uint32_t>(k * sizeof(S))));
+            for (int k = 0; k < R; ++k) {
+                const uint32_t hx = static_cast<uint32_t>(min(c.hx + k, c.p.nx - 1));
+                P.halo[i][k] = Storage<S>::load(*reinterpret_cast<const S*>(row + hx * sizeof(S)));
+            }
         }
     }
 
@@ -145,7 +147,7 @@ struct Stencil3D {
                 for (int t = 0; t < NT; ++t) {
                     const Tap3 tp = tap3(SHAPE, t);
                     const Plane& P = ring[(KT + R + tp.dz + NP) % NP];
-                    acc = __builtin_fma(tw[t], at(c, P, i + R + tp.dy, j, tp.dx), acc);
+                    acc = fmaa(tw[t], at(c, P, i + R + tp.dy, j, tp.dx), acc);
                 }
                 const int x = c.x0 + j;
                 const bool in = zin && yin && x >= R && x < c.p.nx - R;
@@ -212,8 +214,7 @@ struct Stencil3D {
             int hx = c.x0;  // other lanes: any in-range column (their halo is unused)
             if (lane == 0) hx = c.x0 - R;
             if (lane == kWave - 1) hx = c.x0 + V;
-            hx = clampi(hx, 0, p.nx - R);
-            c.hoff = static_cast<uint32_t>(hx) * static_cast<uint32_t>(sizeof(S));
+            c.hx = clampi(hx, 0, p.nx - 1);
         }
 
         Plane ring[NP];
