@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64/BF16 side measurements")
     ap.add_argument("--only", default="", help="comma list of suite kernel names (profiling)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic tiling)")
     return ap.parse_args()
 
 
@@ -79,7 +80,7 @@ def main():
             if events is not None:
                 events[i][0].record(stream)
             api.stencil2d(src, dst, taps(shape, r), shape=L.STAR if shape == "star" else L.BOX,
-                          steps=t, stream=stream)
+                          steps=t, work=work, stream=stream)
             if events is not None:
                 events[i][1].record(stream)
 
@@ -87,8 +88,17 @@ def main():
     src = torch.empty((N, N), dtype=torch.float32, device=dev)
     api.fill_uniform(src, 0, -1.0, 1.0)
     dst = torch.empty_like(src)
+    work = torch.empty_like(src)  # fused steps split over launches ping-pong through it
     torch.cuda.synchronize()
 
+    # autotune each kernel's tiling and temporal-blocking degree on these buffers
+    # (before the timed region; the library caches the choice per request shape)
+    tuning = {}
+    if not a.no_tune:
+        for name, shape, r, t in suite:
+            tuning[name] = api.tune2d(src, dst, taps(shape, r),
+                                      shape=L.STAR if shape == "star" else L.BOX, steps=t, work=work)
+    torch.cuda.synchronize()
     for _ in range(a.warmup):
         run_suite(src, dst)
     torch.cuda.synchronize()
@@ -111,12 +121,14 @@ def main():
     for i, (name, shape, r, t) in enumerate(suite):
         ms = float(np.mean([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(a.steps)]))
         upd = pts * t
+        tw, tc = tuning.get(name, (0, 0))
         byts = 2 * 4 * pts  # each point read once and written once per launch
         taps_n = (4 * r + 1) if shape == "star" else (2 * r + 1) ** 2
         per.append(dict(name=name, ms=round(ms, 4), gpts=round(upd / ms / 1e6, 2),
                         gbps=round(byts / ms / 1e6, 1),
                         hbm_frac=round(byts / ms / 1e6 / HBM_PEAK_GBPS, 3),
-                        gflops=round(upd * (2 * taps_n - 1) / ms / 1e6, 1)))
+                        gflops=round(upd * (2 * taps_n - 1) / ms / 1e6, 1),
+                        tuned_waves_per_simd=tw, tuned_steps_per_launch=tc))
     updates_per_step = sum(pts * t for (_, _, _, t) in suite)
     value = updates_per_step / (ms_per_step * 1e-3) / 1e9
 
@@ -154,6 +166,8 @@ def main():
     extras = {}
     if not a.no_extras:
         extras = side_runs(suite, taps, stream, per)
+        extras["config2_3d"] = side_3d(stream, copy_gbps)
+        extras["config3_conv"] = side_conv(stream, copy_gbps)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -238,6 +252,96 @@ def side_runs(suite, taps, stream, per_f32):
                           kernels=kern)
         del src, dst, work
     return out
+
+
+def _time(fn, stream, reps=3):
+    import torch
+    fn()  # warm
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def side_3d(stream, copy_gbps):
+    """512^3: 7-pt and 13-pt star sweeps (reference coefficients), the reference
+    19-pt 'poisson', and 100 Jacobi sweeps of the 19-point Laplacian with the FP64
+    residual of the last sweep -- FP32 and FP64."""
+    import torch
+
+    from ssam_amd import _lib as L
+    from ssam_amd import api
+    from ssam_amd import workloads as wl
+
+    n = 512
+    out = {}
+    for label, dt, es in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+        src = torch.empty((n, n, n), dtype=dt, device="cuda")
+        api.fill_uniform(src, 2, 0.0, 1.0)
+        dst = torch.empty_like(src)
+        work = torch.empty_like(src)
+        kern = []
+        for name, shape, r, coef in (("7pt", L.STAR, 1, wl.REF3D_7PT), ("13pt", L.STAR, 2, wl.REF3D_13PT),
+                                     ("poisson19", L.FACE_EDGE, 1, wl.REF3D_POISSON)):
+            ms = _time(lambda: api.stencil3d(src, dst, coef, shape=shape, radius=r, stream=stream), stream)
+            gbps = 2 * es * n ** 3 / ms / 1e6
+            kern.append(dict(name=name, ms=round(ms, 4), gpts=round(n ** 3 / ms / 1e6, 2),
+                             gbps=round(gbps, 1), stream_frac=round(gbps / copy_gbps, 3)))
+        steps = 100
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = api.jacobi19(src, dst, steps, work, residual=True, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        gbps = 2 * es * n ** 3 / ms / 1e6
+        kern.append(dict(name="jacobi19_x100", ms_per_step=round(ms, 4),
+                         gpts=round(n ** 3 / ms / 1e6, 2), gbps=round(gbps, 1),
+                         stream_frac=round(gbps / copy_gbps, 3), residual_l2_fp64=res))
+        out[label] = kern
+        del src, dst, work
+    return out
+
+
+def side_conv(stream, copy_gbps):
+    """8192^2 image U[0,1), K x K taps U[-1,1)/K^2: FP32 plain vs BF16 / FP16 storage
+    (FP32 accumulation); errors against the FP64 GPU path on the same values."""
+    import torch
+
+    from ssam_amd import _lib as L
+    from ssam_amd import api
+    from ssam_amd import workloads as wl
+
+    n = N
+    x64 = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    api.fill_uniform(x64, 3, 0.0, 1.0)
+    rows = []
+    for k in (3, 5, 7, 9, 15, 19):
+        f = wl.conv_taps(k, seed=50 + k)
+        ref = torch.empty_like(x64)
+        api.stencil2d(x64, ref, f, shape=L.DENSE, stream=stream)
+        row = dict(K=k)
+        for label, dt in (("fp64", torch.float64), ("fp32", torch.float32), ("bf16", torch.bfloat16),
+                          ("fp16", torch.float16)):
+            src = x64 if dt == torch.float64 else x64.to(dt)
+            dst = torch.empty_like(src)
+            ms = _time(lambda: api.stencil2d(src, dst, f, shape=L.DENSE, stream=stream), stream)
+            ent = dict(ms=round(ms, 4), gpts=round(n * n / ms / 1e6, 2))
+            if dt != torch.float64:
+                d = (dst.to(torch.float64) - ref).abs()
+                ent["max_abs_err"] = float(d.max())
+                ent["rms_err"] = float(torch.sqrt((d * d).mean()))
+                ent["max_rel_err_normwise"] = float(d.max() / ref.abs().max())
+            row[label] = ent
+            del dst
+        row["bf16_speedup_vs_fp32"] = round(row["fp32"]["ms"] / row["bf16"]["ms"], 3)
+        rows.append(row)
+        del ref
+    return rows
 
 
 def cpu_baseline(suite, taps):

@@ -188,6 +188,25 @@ int ssam_poisson_f64(const double* src, double* dst, int n, int iters, double* w
 int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz,
                   int steps, void* work, double* residual_l2, void* stream);
 
+/* ---------------------------------------------------------------- autotuner
+ * Times candidate tilings of one 2D request on the caller's buffers (dst and work
+ * are overwritten; the stream is synchronised): tile height, as the number of
+ * waves per SIMD the tiling targets (2, 4, 8, 12), and the fused steps per launch
+ * (temporal-blocking degree, 8/4/2/1 where the shape fuses and `work` is given).
+ * The best form is cached in the process, keyed by (storage, shape, radius,
+ * steps, width, height), and used by every later ssam_stencil2d call with that
+ * key.  best_* (may be NULL) receive the choice.  No counterpart in the reference,
+ * whose tile parameters are compile-time tables keyed by GPU name
+ * (tmp3d7pt-v2.cu:341-431). */
+int ssam_tune_stencil2d(const ssam_stencil2d_desc* desc, const void* src, void* dst,
+                        int64_t width, int64_t height, int64_t pitch, void* work, void* stream,
+                        int* best_waves_per_simd, int* best_tchunk);
+void ssam_tune_clear(void);
+/* Persist / restore the cache as text lines; load returns SSAM_E_INVALID if the
+ * file holds no valid entry. */
+int ssam_tune_save(const char* path);
+int ssam_tune_load(const char* path);
+
 /* ---------------------------------------------------------------- utilities
  * Seeded synthetic input: element i = a + (b-a) * (splitmix64_{seed}(offset+i) >> 40)
  * * 2^-24, written in `dtype`.  Same values as the CPU generator of the oracle. */

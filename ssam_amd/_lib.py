@@ -81,6 +81,11 @@ _PROTOS = {
     "ssam_poisson_f32": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "ssam_poisson_f64": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "ssam_jacobi19": (_i, [_i, _vp, _vp, _i64, _i64, _i64, _i, _vp, ctypes.POINTER(_d), _vp]),
+    "ssam_tune_stencil2d": (_i, [ctypes.POINTER(Stencil2DDesc), _vp, _vp, _i64, _i64, _i64, _vp, _vp,
+                                 ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "ssam_tune_clear": (None, []),
+    "ssam_tune_save": (_i, [ctypes.c_char_p]),
+    "ssam_tune_load": (_i, [ctypes.c_char_p]),
     "ssam_fill_uniform": (_i, [_vp, _i, _i64, _u64, _u64, _d, _d, _vp]),
     "ssam_stream_copy": (_i, [_vp, _vp, _i64, _vp]),
 }
@@ -118,4 +123,4 @@ def declared_symbols() -> list[str]:
     """Every function name include/ssam.h declares."""
     import re
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(ssam_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(ssam_\w+)\s*\(", text, re.M)))

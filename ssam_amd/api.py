@@ -74,6 +74,41 @@ def stencil2d(src: torch.Tensor, dst: torch.Tensor, coeffs, shape: int = L.BOX, 
     check(rc, "ssam_stencil2d")
 
 
+def tune2d(src: torch.Tensor, dst: torch.Tensor, coeffs, shape: int = L.BOX, steps: int = 1,
+           boundary: int = L.ZERO, work: Optional[torch.Tensor] = None, flags: int = 0,
+           stream: Optional[torch.cuda.Stream] = None) -> tuple[int, int]:
+    """Autotune the tiling of this request (ssam_tune_stencil2d): returns
+    (waves_per_simd, fused_steps_per_launch); later stencil2d calls with the same
+    (storage, shape, radius, steps, width, height) use it.  dst/work are overwritten."""
+    w, h, pitch = _check_2d(src, dst)
+    c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
+    d = L.Stencil2DDesc()
+    d.storage = dtype_code(src.dtype)
+    d.shape = shape
+    d.fh, d.fw = c.shape
+    d.steps = steps
+    d.boundary = boundary
+    d.flags = flags
+    d.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    wps, tch = ctypes.c_int(0), ctypes.c_int(0)
+    check(L.lib().ssam_tune_stencil2d(ctypes.byref(d), _dev(src, "src"), _dev(dst, "dst"), w, h,
+                                      pitch, _dev(work, "work"), _stream(stream), ctypes.byref(wps),
+                                      ctypes.byref(tch)), "ssam_tune_stencil2d")
+    return wps.value, tch.value
+
+
+def tune_save(path: str) -> None:
+    check(L.lib().ssam_tune_save(str(path).encode()), "ssam_tune_save")
+
+
+def tune_load(path: str) -> None:
+    check(L.lib().ssam_tune_load(str(path).encode()), "ssam_tune_load")
+
+
+def tune_clear() -> None:
+    L.lib().ssam_tune_clear()
+
+
 def _sfx(t: torch.Tensor) -> str:
     if t.dtype == torch.float32:
         return "f32"

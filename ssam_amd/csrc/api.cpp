@@ -5,8 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -59,7 +61,46 @@ int max_fused(int storage, int shape, int radius) {
     return kMaxFusedSteps;
 }
 
+// ------------------------------------------------------------------ autotuner cache
+// Per (storage, shape, radius, steps, width, height): the tile height target
+// (waves per SIMD the tiling aims for) and the fused steps per launch.  Filled by
+// ssam_tune_stencil2d (timed candidates on the caller's buffers), read by every
+// ssam_stencil2d call; save/load as text lines.
+struct TuneKey {
+    int storage, shape, radius, steps;
+    int64_t w, h;
+    bool operator<(const TuneKey& o) const {
+        if (storage != o.storage) return storage < o.storage;
+        if (shape != o.shape) return shape < o.shape;
+        if (radius != o.radius) return radius < o.radius;
+        if (steps != o.steps) return steps < o.steps;
+        if (w != o.w) return w < o.w;
+        return h < o.h;
+    }
+};
+struct TuneVal {
+    int waves_per_simd;  // tile-count target
+    int tchunk;          // fused steps per launch (<= kernel maximum)
+    float ms;            // measured time of the chosen form
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+constexpr int kDefaultWavesPerSimd = 8;
+
+bool tune_lookup(const TuneKey& k, TuneVal* v) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(k);
+    if (it == g_tune.end()) return false;
+    *v = it->second;
+    return true;
+}
+
 }  // namespace
+
+// Forced tiling for one call (0 = cache / heuristic); used by the tuner.
+thread_local int t_force_wps = 0;
+thread_local int t_force_tchunk = 0;
 
 int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int64_t width,
                    int64_t height, int64_t pitch, void* work, hipStream_t st) {
@@ -114,7 +155,12 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
     if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
 
     // split the steps into fused launches (8, 4, 2, 1) chained through work/dst
-    const int fmax = max_fused(d->storage, shape, R);
+    const TuneKey key{d->storage, kshape, R, d->steps, width, height};
+    TuneVal tv{kDefaultWavesPerSimd, 0, 0.f};
+    const bool tuned = tune_lookup(key, &tv);
+    int fmax = max_fused(d->storage, shape, R);
+    if (t_force_tchunk > 0) fmax = t_force_tchunk < fmax ? t_force_tchunk : fmax;
+    else if (tuned && tv.tchunk > 0 && tv.tchunk < fmax) fmax = tv.tchunk;
     std::vector<int> chunks;
     for (int left = d->steps; left > 0;) {
         int t = fmax;
@@ -137,7 +183,9 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
     rq.boundary = d->boundary;
     rq.taps = taps.data();
     rq.seg_rows = env_int("SSAM_SEG_ROWS", 0);
-    rq.waves_per_simd = env_int("SSAM_WAVES_PER_SIMD", 4);
+    rq.waves_per_simd = t_force_wps > 0 ? t_force_wps
+                        : env_int("SSAM_WAVES_PER_SIMD", tuned ? tv.waves_per_simd
+                                                               : kDefaultWavesPerSimd);
     const size_t n = chunks.size();
     const void* in = src;
     for (size_t i = 0; i < n; ++i) {
@@ -482,6 +530,128 @@ int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t n
     // r = (2F + E - 24u)/6 = 4 * (u' - u) with u' = (2F + E)/24
     return stencil3d_run(storage, SSAM_SHAPE_FACE_EDGE, 1, kJacobi19, steps, src, dst, nx, ny, nz,
                          work, residual_l2, 4.0, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ autotuner
+namespace {
+
+struct HipEvents {
+    hipEvent_t a = nullptr, b = nullptr;
+    HipEvents() {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+    }
+    ~HipEvents() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst, int64_t width,
+                        int64_t height, int64_t pitch, void* work, void* stream,
+                        int* best_waves_per_simd, int* best_tchunk) {
+    if (!d) return SSAM_E_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // validate once through the normal path (also warms the code object)
+    int rc = stencil2d_impl(d, src, dst, width, height, pitch, work, st);
+    if (rc != SSAM_OK) return rc;
+    int R = d->fw / 2 > d->fh / 2 ? d->fw / 2 : d->fh / 2;
+    if (R < 1) R = 1;
+    const int kshape = d->shape == SSAM_SHAPE_STAR ? 0 : 1;
+    const int fmax = max_fused(d->storage, d->shape, R);
+    const int wps_c[] = {2, 4, 8, 12};
+    std::vector<int> tch_c;
+    for (int t = fmax; t >= 1; t >>= 1) {
+        if (d->steps > 1 && t > d->steps) continue;
+        const bool multi = (d->steps + t - 1) / t > 1;
+        if (multi && !work) continue;
+        tch_c.push_back(t);
+        if (d->steps == 1) break;
+    }
+    HipEvents ev;
+    if (!ev.a || !ev.b) return SSAM_E_HIP;
+    TuneVal best{kDefaultWavesPerSimd, 0, 1e30f};
+    for (int tc : tch_c) {
+        for (int wps : wps_c) {
+            t_force_wps = wps;
+            t_force_tchunk = tc;
+            float ms_sum = 0.f;
+            bool ok = true;
+            rc = stencil2d_impl(d, src, dst, width, height, pitch, work, st);  // warm
+            for (int rep = 0; rep < 3 && rc == SSAM_OK; ++rep) {
+                if (hipEventRecord(ev.a, st) != hipSuccess) { ok = false; break; }
+                rc = stencil2d_impl(d, src, dst, width, height, pitch, work, st);
+                if (hipEventRecord(ev.b, st) != hipSuccess || hipEventSynchronize(ev.b) != hipSuccess) {
+                    ok = false;
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, ev.a, ev.b);
+                ms_sum += ms;
+            }
+            t_force_wps = 0;
+            t_force_tchunk = 0;
+            if (!ok) return SSAM_E_HIP;
+            if (rc != SSAM_OK) return rc;
+            if (ms_sum / 3 < best.ms) best = TuneVal{wps, tc, ms_sum / 3};
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[TuneKey{d->storage, kshape, R, d->steps, width, height}] = best;
+    }
+    if (best_waves_per_simd) *best_waves_per_simd = best.waves_per_simd;
+    if (best_tchunk) *best_tchunk = best.tchunk;
+    return SSAM_OK;
+}
+
+void ssam_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
+int ssam_tune_save(const char* path) {
+    if (!path) return SSAM_E_INVALID;
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return SSAM_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::fprintf(fp, "# ssam autotune v1: storage shape radius steps width height waves_per_simd tchunk ms\n");
+    for (const auto& kv : g_tune)
+        std::fprintf(fp, "%d %d %d %d %lld %lld %d %d %.6f\n", kv.first.storage, kv.first.shape,
+                     kv.first.radius, kv.first.steps, static_cast<long long>(kv.first.w),
+                     static_cast<long long>(kv.first.h), kv.second.waves_per_simd, kv.second.tchunk,
+                     static_cast<double>(kv.second.ms));
+    std::fclose(fp);
+    return SSAM_OK;
+}
+
+int ssam_tune_load(const char* path) {
+    if (!path) return SSAM_E_INVALID;
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return SSAM_E_INVALID;
+    char line[512];
+    int n = 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    while (std::fgets(line, sizeof(line), fp)) {
+        if (line[0] == '#') continue;
+        int sto, sh, r, steps, wps, tc;
+        long long w, h;
+        double ms;
+        if (std::sscanf(line, "%d %d %d %d %lld %lld %d %d %lf", &sto, &sh, &r, &steps, &w, &h, &wps,
+                        &tc, &ms) == 9 &&
+            wps > 0 && wps <= 64 && tc >= 0) {
+            g_tune[TuneKey{sto, sh, r, steps, w, h}] = TuneVal{wps, tc, static_cast<float>(ms)};
+            ++n;
+        }
+    }
+    std::fclose(fp);
+    return n > 0 ? SSAM_OK : SSAM_E_INVALID;
 }
 
 }  // extern "C"

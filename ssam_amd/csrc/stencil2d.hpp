@@ -163,6 +163,7 @@ struct Stencil2D {
         S* __restrict__ dst;
         Params2D p;
         int x0, ys, ye;
+        int ylast;      // last input row this tile needs (ye - 1 + T*R)
         uint32_t xoff;  // byte offset of this lane's strip within a row (x0 >= 0)
         bool lane_full, edge, store_lane;
         bool inb[V];
@@ -233,7 +234,9 @@ struct Stencil2D {
                                                 A (&ring0)[NR0][V],
                                                 A (&rings)[NSTAGE_RINGS][NRS][V]) {
         const int yc = yc0 + KT;
-        if constexpr (RING) load_row<CHK>(L, yc + PF, ring0[s0(KT, PF + 2 * R)]);
+        // rows past the tile's last needed row only feed discarded ticks: re-read
+        // that row instead (an L2 hit) so no HBM bytes are fetched for them
+        if constexpr (RING) load_row<CHK>(L, min(yc + PF, L.ylast), ring0[s0(KT, PF + 2 * R)]);
         // taps come from the kernel-argument segment through scalar loads; the
         // opaque zero offset keeps them from being hoisted out of the row loop
         // into (uniform) VGPRs, which large boxes cannot afford
@@ -310,12 +313,13 @@ struct Stencil2D {
             for (int j = 0; j < V; ++j) ring0[i][j] = 0;
         // rows c0-2R .. c0+PF-1 into slots 0 .. 2R+PF-1
 #pragma unroll
-        for (int s = 0; s < 2 * R + PF; ++s) load_row<CHK>(L, c0 - 2 * R + s, ring0[s]);
+        for (int s = 0; s < 2 * R + PF; ++s) load_row<CHK>(L, min(c0 - 2 * R + s, L.ylast), ring0[s]);
         int yc0 = c0;
         for (int b = 0; b < nblk; ++b, yc0 += U) {
             if constexpr (!RING) {
 #pragma unroll
-                for (int k = 0; k < U; ++k) load_row<CHK>(L, yc0 + k + PF, ring0[2 * R + PF + k]);
+                for (int k = 0; k < U; ++k)
+                    load_row<CHK>(L, min(yc0 + k + PF, L.ylast), ring0[2 * R + PF + k]);
             }
             block<CHK, 0>(L, w, yc0, ring0, rings);
             if constexpr (!RING) {
@@ -363,8 +367,9 @@ struct Stencil2D {
         const int c0 = L.ys - T * R + 2 * R;
         const int ntick = (L.ye - 1 + T * R) - c0 + 1;
         const int nblk = (ntick + U - 1) / U;
+        L.ylast = L.ye - 1 + T * R;
         const int y_lo = c0 - 2 * R;
-        const int y_hi = c0 + nblk * U - 1 + PF;  // last row any tick loads
+        const int y_hi = min(c0 + nblk * U - 1 + PF, L.ylast);  // last row any tick loads
         const bool interior = !L.edge && p.vec_ok && y_lo >= 0 && y_hi < p.height;
         if (interior)
             stream<false>(L, w, c0, nblk);

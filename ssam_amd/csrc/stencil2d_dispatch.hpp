@@ -31,8 +31,8 @@ template <int R, int SHAPE, int V, int T> struct Unroll2D {
     static constexpr int NR0 = K + PF_RING;
     static constexpr int PERIOD = T > 1 ? (NR0 / cgcd(NR0, K)) * K : NR0;
     static constexpr bool RING = PERIOD * BODY <= 2048;
-    static constexpr int U0 = 1024 / BODY;
-    static constexpr int U = RING ? PERIOD : (U0 < 1 ? 1 : (U0 > 4 ? 4 : U0));
+    static constexpr int U0 = 2048 / BODY;
+    static constexpr int U = RING ? PERIOD : (U0 < 1 ? 1 : (U0 > 6 ? 6 : U0));
     static constexpr int PF = RING ? PF_RING : (U > 3 ? U : 3);
     static constexpr int WPE = 2;
 };

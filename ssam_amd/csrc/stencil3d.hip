@@ -86,6 +86,9 @@ struct Stencil3D {
         bool full;
     };
 
+    // CHK = false: every lane of the wave holds a full, aligned strip, so rows load
+    // with one unconditional vector access (no per-lane branch in the hot loop).
+    template <bool CHK>
     __device__ static __forceinline__ void load_plane(const Ctx& c, int z, Plane& P) {
         const int zz = clampi(z, 0, c.p.nz - 1);
         const char* base = reinterpret_cast<const char*>(c.src) +
@@ -94,7 +97,7 @@ struct Stencil3D {
         for (int i = 0; i < NROW; ++i) {
             const int y = clampi(c.y0 - R + i, 0, c.p.ny - 1);
             const char* row = base + static_cast<long long>(y) * c.p.sy * static_cast<long long>(sizeof(S));
-            if (c.full && c.p.vec_ok) {
+            if (!CHK || (c.full && c.p.vec_ok)) {
                 load_strip_vec<S, V>(reinterpret_cast<const S*>(row + c.xoff), P.v[i]);
             } else {
 #pragma unroll
@@ -125,12 +128,12 @@ struct Stencil3D {
         return c.lane == kWave - 1 ? P.halo[i][k] : d;
     }
 
-    template <int KT>
+    template <bool CHK, int KT>
     __device__ static __forceinline__ void tick(const Ctx& c, const A* __restrict__ w, int zq0,
                                                 Plane (&ring)[NP], double& res) {
         const int q = zq0 + KT;  // output plane
         // prefetch plane q + R + PF into the slot of plane q - R - 1
-        load_plane(c, q + R + PF, ring[(KT + 2 * R + PF) % NP]);
+        load_plane<CHK>(c, q + R + PF, ring[(KT + 2 * R + PF) % NP]);
         int zero = 0;
         asm volatile("" : "+s"(zero));
         const A* __restrict__ tw = w + zero;
@@ -162,7 +165,7 @@ struct Stencil3D {
                 char* row = reinterpret_cast<char*>(c.dst) +
                             (static_cast<long long>(q) * c.p.sz + static_cast<long long>(y) * c.p.sy) *
                                 static_cast<long long>(sizeof(S));
-                if (c.full && c.p.vec_ok) {
+                if (!CHK || (c.full && c.p.vec_ok)) {
                     store_strip_vec<S, V>(reinterpret_cast<S*>(row + c.xoff), out);
                 } else {
 #pragma unroll
@@ -175,13 +178,25 @@ struct Stencil3D {
         }
     }
 
-    template <int KT>
+    template <bool CHK, int KT>
     __device__ static __forceinline__ void period(const Ctx& c, const A* __restrict__ w, int zq0,
                                                   int zend, Plane (&ring)[NP], double& res) {
         if constexpr (KT < NP) {
-            if (zq0 + KT < zend) tick<KT>(c, w, zq0, ring, res);
-            period<KT + 1>(c, w, zq0, zend, ring, res);
+            if (zq0 + KT < zend) tick<CHK, KT>(c, w, zq0, ring, res);
+            period<CHK, KT + 1>(c, w, zq0, zend, ring, res);
         }
+    }
+
+    template <bool CHK>
+    __device__ static __forceinline__ double stream(const Ctx& c, const A* __restrict__ w, int zs,
+                                                    int ze) {
+        Plane ring[NP];
+        // planes zs-R .. zs+R+PF-1 into the slots of the first period
+#pragma unroll
+        for (int k = 0; k < 2 * R + PF; ++k) load_plane<CHK>(c, zs - R + k, ring[k]);
+        double res = 0.0;
+        for (int zq0 = zs; zq0 < ze; zq0 += NP) period<CHK, 0>(c, w, zq0, ze, ring, res);
+        return res;
     }
 
     __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const Params3D& p,
@@ -217,12 +232,9 @@ struct Stencil3D {
             c.hx = clampi(hx, 0, p.nx - 1);
         }
 
-        Plane ring[NP];
-        // planes zs-R .. zs+R+PF-1 into the slots of the first period
-#pragma unroll
-        for (int k = 0; k < 2 * R + PF; ++k) load_plane(c, zs - R + k, ring[k]);
-        double res = 0.0;
-        for (int zq0 = zs; zq0 < ze; zq0 += NP) period<0>(c, w, zq0, ze, ring, res);
+        // the strip is full for every lane when its last column is inside the grid
+        const bool interior = p.vec_ok && (strip + 1) * kWave * V <= p.nx;
+        double res = interior ? stream<false>(c, w, zs, ze) : stream<true>(c, w, zs, ze);
 
         if (p.partials) {
             // wave reduction in a fixed order, then one partial per tile

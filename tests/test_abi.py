@@ -123,3 +123,24 @@ def test_utilities_validate():
     assert lib.ssam_stream_copy(FAKE_A, FAKE_B, 10, None) == L.E_INVALID
     assert lib.ssam_stream_copy(ctypes.c_void_p(0x10000004), FAKE_B, 16, None) == L.E_INVALID
     assert lib.ssam_stream_copy(FAKE_A, FAKE_B, 0, None) == L.OK
+
+
+def test_tuner_cache_io(tmp_path):
+    lib = L.lib()
+    lib.ssam_tune_clear()
+    assert lib.ssam_tune_load(str(tmp_path / "missing.txt").encode()) == L.E_INVALID
+    assert lib.ssam_tune_load(None) == L.E_INVALID
+    p = tmp_path / "tune.txt"
+    assert lib.ssam_tune_save(str(p).encode()) == L.OK
+    assert p.read_text().startswith("# ssam autotune v1")
+    assert lib.ssam_tune_load(str(p).encode()) == L.E_INVALID   # header only: no entry
+    p.write_text("0 0 1 1 8192 8192 8 0 0.125\n1 1 5 1 64 64 4 0 0.5\nbad line\n")
+    assert lib.ssam_tune_load(str(p).encode()) == L.OK
+    q = tmp_path / "again.txt"
+    assert lib.ssam_tune_save(str(q).encode()) == L.OK
+    lines = [l for l in q.read_text().splitlines() if not l.startswith("#")]
+    assert len(lines) == 2
+    lib.ssam_tune_clear()
+    d, c = _desc()
+    assert lib.ssam_tune_stencil2d(None, FAKE_A, FAKE_B, 64, 64, 64, None, None, None, None) == L.E_INVALID
+    assert lib.ssam_tune_stencil2d(ctypes.byref(d), FAKE_A, FAKE_A, 64, 64, 64, None, None, None, None) == L.E_INVALID
