@@ -168,6 +168,7 @@ def main():
         extras = side_runs(suite, taps, stream, per)
         extras["config2_3d"] = side_3d(stream, copy_gbps)
         extras["config3_conv"] = side_conv(stream, copy_gbps)
+        extras["config4_batch"] = side_batch(stream)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -342,6 +343,84 @@ def side_conv(stream, copy_gbps):
         rows.append(row)
         del ref
     return rows
+
+
+def side_batch(stream):
+    """The uneven batch: 192 2D grids (5-pt star / 5x5 / 11x11
+    box) in one persistent work-queue launch plus 64 3D grids x 10 sweeps of the
+    7-point star, replayed as one hipGraph.  Efficiency = the time the same point
+    counts would take at single-large-grid throughput (8192^2 per 2D shape, 512^3
+    for the 7-point star, measured here) / the batch time."""
+    import torch
+
+    from ssam_amd import _lib as L
+    from ssam_amd import api
+    from ssam_amd import workloads as wl
+
+    g2, g3 = wl.config4_batch(seed=4)
+    shapes = {("star", 1): L.STAR, ("box", 2): L.BOX, ("box", 5): L.BOX}
+    out = {}
+    # single-large-grid reference throughputs (Gpoints/s)
+    ref = {}
+    for dt, lab in ((torch.float32, "fp32"), (torch.bfloat16, "bf16")):
+        big = torch.empty((N, N), dtype=torch.float32, device="cuda")
+        api.fill_uniform(big, 5, -1.0, 1.0)
+        big = big.to(dt)
+        bd = torch.empty_like(big)
+        for (sh, r), code in shapes.items():
+            f = wl.normalised_taps(sh, r, 60 + r)
+            ms = _time(lambda: api.stencil2d(big, bd, f, shape=code, stream=stream), stream)
+            ref[(lab, sh, r)] = N * N / ms / 1e6
+        del big, bd
+    n3 = 512
+    b3 = torch.empty((n3, n3, n3), dtype=torch.float32, device="cuda")
+    api.fill_uniform(b3, 6, 0.0, 1.0)
+    d3 = torch.empty_like(b3)
+    ms = _time(lambda: api.stencil3d(b3, d3, wl.REF3D_7PT, shape=L.STAR, radius=1, stream=stream), stream)
+    ref3 = n3 ** 3 / ms / 1e6
+    del b3, d3
+    for dt, lab in ((torch.float32, "fp32"), (torch.bfloat16, "bf16")):
+        items, keep = [], []
+        exp_ms = 0.0
+        pts2 = pts3 = 0
+        for k, g in enumerate(g2):
+            src = torch.empty((g["h"], g["w"]), dtype=torch.float32, device="cuda")
+            api.fill_uniform(src, 1000 + k, -1.0, 1.0)
+            src = src.to(dt)
+            dst = torch.empty_like(src)
+            keep += [src, dst]
+            items.append(dict(ndim=2, src=src, dst=dst, shape=shapes[(g["shape"], g["r"])],
+                              radius=g["r"], coeffs=wl.normalised_taps(g["shape"], g["r"], 60 + g["r"])))
+            pts2 += g["w"] * g["h"]
+            exp_ms += g["w"] * g["h"] / (ref[(lab, g["shape"], g["r"])] * 1e6)
+        for k, g in enumerate(g3):
+            src = torch.empty((g["nz"], g["ny"], g["nx"]), dtype=torch.float32, device="cuda")
+            api.fill_uniform(src, 5000 + k, 0.0, 1.0)
+            dst, work = torch.empty_like(src), torch.empty_like(src)
+            keep += [src, dst, work]
+            items.append(dict(ndim=3, src=src, dst=dst, work=work, shape=L.STAR, radius=1,
+                              steps=g["steps"], coeffs=wl.REF3D_7PT))
+            n = g["nx"] * g["ny"] * g["nz"] * g["steps"]
+            pts3 += n
+            exp_ms += n / (ref3 * 1e6)
+        plan = api.BatchPlan(items)
+        t2, t3, pu = plan.info()
+        plan.run(stream=stream)  # captures the graph
+        torch.cuda.synchronize()
+        ms = _time(lambda: plan.run(stream=stream), stream, reps=5)
+        ms_eager = _time(lambda: plan.run(eager=True, stream=stream), stream, reps=3)
+        out[lab] = dict(ms=round(ms, 4), ms_eager=round(ms_eager, 4), point_updates=pu,
+                        gpts=round(pu / ms / 1e6, 2), tiles_2d=t2, tiles_3d=t3,
+                        points_2d=pts2, point_updates_3d=pts3,
+                        single_grid_equivalent_ms=round(exp_ms, 4),
+                        efficiency_vs_single_grid=round(exp_ms / ms, 3),
+                        storage_2d=lab, storage_3d="fp32")
+        plan.close()
+        del keep, items
+        torch.cuda.empty_cache()
+    out["single_grid_gpts"] = {f"{k[0]}_{k[1]}{k[2]}": round(v, 1) for k, v in ref.items()}
+    out["single_grid_gpts"]["fp32_3d7pt"] = round(ref3, 1)
+    return out
 
 
 def cpu_baseline(suite, taps):

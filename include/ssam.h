@@ -188,6 +188,39 @@ int ssam_poisson_f64(const double* src, double* dst, int n, int iters, double* w
 int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz,
                   int steps, void* work, double* residual_l2, void* stream);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent grids of uneven sizes and mixed shapes.  2D items of one
+ * storage type whose shape is a 5-point star or a 3x3 / 5x5 / 11x11 box run as
+ * ONE persistent launch whose waves take tiles from a device work queue (longest
+ * tiles first); 3D items (7-point star or 19-point face+edge, F32/F64) run one
+ * persistent launch per sweep; any other item gets its own launch.  The first
+ * ssam_batch_run captures the whole sequence into a hipGraph on a private stream
+ * and every run replays it on `stream` (SSAM_BATCH_EAGER: launch directly).
+ * Device pointers are bound at plan creation.  No counterpart in the reference. */
+typedef struct {
+    int32_t ndim;          /* 2 or 3 */
+    int32_t storage;       /* ssam_dtype (3D: F32 or F64) */
+    int32_t shape;         /* 2D: STAR or BOX; 3D: STAR or FACE_EDGE */
+    int32_t radius;        /* filter radius r */
+    int32_t steps;         /* 2D: 1 in the persistent launch; 3D: sweeps */
+    int32_t reserved;
+    int64_t nx, ny, nz;    /* 2D: width, height, 1 */
+    int64_t pitch;         /* 2D row pitch in elements; 3D grids are dense */
+    const void* src;
+    void* dst;
+    void* work;            /* 3D with steps > 1: ping-pong buffer of the same size */
+    const double* coeffs;  /* HOST: 2D (2r+1)^2 row-major; 3D descriptor tap order */
+} ssam_batch_item;
+
+typedef struct ssam_batch_plan_s* ssam_batch_plan;
+#define SSAM_BATCH_EAGER 1u
+
+int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan* plan);
+int ssam_batch_run(ssam_batch_plan plan, void* stream, uint32_t flags);
+int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles_3d,
+                         int64_t* point_updates);
+void ssam_batch_plan_destroy(ssam_batch_plan plan);
+
 /* ---------------------------------------------------------------- autotuner
  * Times candidate tilings of one 2D request on the caller's buffers (dst and work
  * are overwritten; the stream is synchronised): tile height, as the number of

@@ -50,6 +50,27 @@ class Stencil3DDesc(ctypes.Structure):
     ]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int32),
+        ("storage", ctypes.c_int32),
+        ("shape", ctypes.c_int32),
+        ("radius", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("nx", ctypes.c_int64),
+        ("ny", ctypes.c_int64),
+        ("nz", ctypes.c_int64),
+        ("pitch", ctypes.c_int64),
+        ("src", ctypes.c_void_p),
+        ("dst", ctypes.c_void_p),
+        ("work", ctypes.c_void_p),
+        ("coeffs", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+BATCH_EAGER = 1
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -86,6 +107,11 @@ _PROTOS = {
     "ssam_tune_clear": (None, []),
     "ssam_tune_save": (_i, [ctypes.c_char_p]),
     "ssam_tune_load": (_i, [ctypes.c_char_p]),
+    "ssam_batch_plan_create": (_i, [ctypes.POINTER(BatchItem), _i, ctypes.POINTER(ctypes.c_void_p)]),
+    "ssam_batch_run": (_i, [ctypes.c_void_p, _vp, ctypes.c_uint32]),
+    "ssam_batch_plan_info": (_i, [ctypes.c_void_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                  ctypes.POINTER(_i64)]),
+    "ssam_batch_plan_destroy": (None, [ctypes.c_void_p]),
     "ssam_fill_uniform": (_i, [_vp, _i, _i64, _u64, _u64, _d, _d, _vp]),
     "ssam_stream_copy": (_i, [_vp, _vp, _i64, _vp]),
 }

@@ -247,3 +247,60 @@ def stream_copy(src: torch.Tensor, dst: torch.Tensor, stream=None) -> None:
     nbytes = src.numel() * src.element_size()
     check(L.lib().ssam_stream_copy(_dev(src, "src"), _dev(dst, "dst"), nbytes, _stream(stream)),
           "ssam_stream_copy")
+
+
+class BatchPlan:
+    """Many grids in one plan (ssam_batch_plan_create / ssam_batch_run).
+
+    items: dicts with keys ndim (2/3), src, dst, coeffs, shape, radius, and for 3D
+    steps and work.  2D tensors are (h, w) with contiguous rows; 3D tensors are
+    contiguous (nz, ny, nx).  The plan binds the device pointers; keep the tensors
+    alive while it exists."""
+
+    def __init__(self, items):
+        self._keep = []
+        arr = (L.BatchItem * len(items))()
+        for k, it in enumerate(items):
+            src, dst = it["src"], it["dst"]
+            b = arr[k]
+            b.ndim = it["ndim"]
+            b.storage = dtype_code(src.dtype)
+            b.shape = it["shape"]
+            b.radius = it["radius"]
+            b.steps = it.get("steps", 1)
+            if b.ndim == 2:
+                h, w = src.shape
+                b.nx, b.ny, b.nz, b.pitch = w, h, 1, src.stride(0)
+            else:
+                nz, ny, nx = src.shape
+                b.nx, b.ny, b.nz, b.pitch = nx, ny, nz, 0
+            b.src = _dev(src, "src").value
+            b.dst = _dev(dst, "dst").value
+            b.work = _dev(it.get("work"), "work").value
+            c = np.ascontiguousarray(np.asarray(it["coeffs"], dtype=np.float64).ravel())
+            self._keep.append(c)
+            b.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h = ctypes.c_void_p()
+        check(L.lib().ssam_batch_plan_create(arr, len(items), ctypes.byref(h)), "ssam_batch_plan_create")
+        self._h = h
+
+    def run(self, eager: bool = False, stream=None) -> None:
+        check(L.lib().ssam_batch_run(self._h, _stream(stream), L.BATCH_EAGER if eager else 0),
+              "ssam_batch_run")
+
+    def info(self) -> tuple[int, int, int]:
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(L.lib().ssam_batch_plan_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+              "ssam_batch_plan_info")
+        return a.value, b.value, c.value
+
+    def close(self) -> None:
+        if self._h:
+            L.lib().ssam_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

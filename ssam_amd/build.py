@@ -42,6 +42,7 @@ SOURCES = [
     "stencil2d_bf16.hip",
     "stencil2d_f16.hip",
     "stencil3d.hip",
+    "batch.hip",
 ]
 
 

@@ -337,16 +337,12 @@ struct Stencil2D {
         }
     }
 
-    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const Params2D& p,
-                               const A* __restrict__ w) {
+    // One tile: column strip `strip`, output rows [seg*seg_rows, ...) of the grid
+    // described by p.  strip and seg are wave-uniform.
+    __device__ static __forceinline__ void run_tile(const S* __restrict__ src, S* __restrict__ dst,
+                                                    const Params2D& p, const A* __restrict__ w,
+                                                    int strip, int seg) {
         const int lane = threadIdx.x & (kWave - 1);
-        // wave index is uniform across the wave: keep the tile bookkeeping in SGPRs
-        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-        const int tile = xcd_remap(blockIdx.x, p.nblocks) * kWavesPerBlock2D + wv;
-        if (tile >= p.ntiles) return;  // whole wave leaves; no barriers are used
-        const int strip = tile % p.nstrips;
-        const int seg = tile / p.nstrips;
-
         Lane L;
         L.src = src;
         L.dst = dst;
@@ -375,6 +371,15 @@ struct Stencil2D {
             stream<false>(L, w, c0, nblk);
         else
             stream<true>(L, w, c0, nblk);
+    }
+
+    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const Params2D& p,
+                               const A* __restrict__ w) {
+        // wave index is uniform across the wave: keep the tile bookkeeping in SGPRs
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+        const int tile = xcd_remap(blockIdx.x, p.nblocks) * kWavesPerBlock2D + wv;
+        if (tile >= p.ntiles) return;  // whole wave leaves; no barriers are used
+        run_tile(src, dst, p, w, tile % p.nstrips, tile / p.nstrips);
     }
 };
 

@@ -91,3 +91,29 @@ def taps3d(shape: str, r: int) -> list[tuple[int, int, int]]:
     for k in range(1, r + 1):
         t += [(0, 0, -k), (0, 0, k), (0, -k, 0), (0, k, 0), (-k, 0, 0), (k, 0, 0)]
     return t
+
+
+def config4_batch(seed: int = 4, n2d: int = 192, n3d: int = 64):
+    """The uneven batch: n2d 2D grids with sides log-uniform in [128, 4096] (aspect
+    ratio at most 4:1, redrawn otherwise), each a 5-point star, 5x5 box or 11x11 box
+    at random; n3d 3D grids with sides log-uniform in [64, 256] running the 7-point
+    star for 10 steps.  Returns (list2d, list3d) of dicts with sizes and shapes; all
+    from one splitmix64 stream."""
+    import math
+    u = iter(uniform(seed, 100000))
+    lo, hi = math.log(128), math.log(4096)
+    g2 = []
+    while len(g2) < n2d:
+        w = int(round(math.exp(lo + (hi - lo) * next(u))))
+        h = int(round(math.exp(lo + (hi - lo) * next(u))))
+        if max(w, h) > 4 * min(w, h):
+            continue
+        kind = min(2, int(3 * next(u)))
+        shape, r = [("star", 1), ("box", 2), ("box", 5)][kind]
+        g2.append(dict(w=w, h=h, shape=shape, r=r))
+    lo3, hi3 = math.log(64), math.log(256)
+    g3 = []
+    for _ in range(n3d):
+        nx, ny, nz = (int(round(math.exp(lo3 + (hi3 - lo3) * next(u)))) for _ in range(3))
+        g3.append(dict(nx=nx, ny=ny, nz=nz, shape="star", r=1, steps=10))
+    return g2, g3

@@ -144,3 +144,28 @@ def test_tuner_cache_io(tmp_path):
     d, c = _desc()
     assert lib.ssam_tune_stencil2d(None, FAKE_A, FAKE_B, 64, 64, 64, None, None, None, None) == L.E_INVALID
     assert lib.ssam_tune_stencil2d(ctypes.byref(d), FAKE_A, FAKE_A, 64, 64, 64, None, None, None, None) == L.E_INVALID
+
+
+def test_batch_plan_validates():
+    lib = L.lib()
+    h = ctypes.c_void_p()
+    assert lib.ssam_batch_plan_create(None, 1, ctypes.byref(h)) == L.E_INVALID
+    arr = (L.BatchItem * 1)()
+    assert lib.ssam_batch_plan_create(arr, 0, ctypes.byref(h)) == L.E_INVALID
+    c = np.ones(9)
+    b = arr[0]
+    b.ndim, b.storage, b.shape, b.radius, b.steps = 2, L.F32, L.STAR, 1, 1
+    b.nx, b.ny, b.nz, b.pitch = 64, 64, 1, 64
+    b.src, b.dst = 0x10000000, 0x10000000   # aliasing
+    b.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
+    b.dst = 0x20000000
+    b.ndim = 4
+    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
+    b.ndim, b.pitch = 2, 10                  # pitch < width
+    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
+    b.ndim, b.nz, b.storage, b.steps, b.pitch = 3, 8, L.BF16, 1, 0
+    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_UNSUPPORTED
+    b.storage, b.steps = L.F32, 4            # 3D multi-step without work
+    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
+    assert lib.ssam_batch_run(None, None, 0) == L.E_INVALID
