@@ -403,6 +403,15 @@ def side_batch(stream):
             n = g["nx"] * g["ny"] * g["nz"] * g["steps"]
             pts3 += n
             exp_ms += n / (ref3 * 1e6)
+        # the 2D and 3D halves alone, to attribute the time
+        parts = {}
+        for part, sel in (("2d_only", lambda it: it["ndim"] == 2), ("3d_only", lambda it: it["ndim"] == 3)):
+            pp = api.BatchPlan([it for it in items if sel(it)])
+            pp.run(stream=stream)
+            torch.cuda.synchronize()
+            parts[part] = round(_time(lambda: pp.run(stream=stream), stream, reps=3), 4)
+            pp.close()
+        exp2 = sum(g["w"] * g["h"] / (ref[(lab, g["shape"], g["r"])] * 1e6) for g in g2)
         plan = api.BatchPlan(items)
         t2, t3, pu = plan.info()
         plan.run(stream=stream)  # captures the graph
@@ -414,6 +423,8 @@ def side_batch(stream):
                         points_2d=pts2, point_updates_3d=pts3,
                         single_grid_equivalent_ms=round(exp_ms, 4),
                         efficiency_vs_single_grid=round(exp_ms / ms, 3),
+                        part_ms=parts, single_grid_equivalent_2d_ms=round(exp2, 4),
+                        single_grid_equivalent_3d_ms=round(exp_ms - exp2, 4),
                         storage_2d=lab, storage_3d="fp32")
         plan.close()
         del keep, items

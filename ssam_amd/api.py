@@ -269,10 +269,10 @@ class BatchPlan:
             b.radius = it["radius"]
             b.steps = it.get("steps", 1)
             if b.ndim == 2:
-                h, w = src.shape
-                b.nx, b.ny, b.nz, b.pitch = w, h, 1, src.stride(0)
+                w, h, pitch = _check_2d(src, dst)
+                b.nx, b.ny, b.nz, b.pitch = w, h, 1, pitch
             else:
-                nz, ny, nx = src.shape
+                nx, ny, nz = _check_3d(src, dst)
                 b.nx, b.ny, b.nz, b.pitch = nx, ny, nz, 0
             b.src = _dev(src, "src").value
             b.dst = _dev(dst, "dst").value

@@ -99,9 +99,11 @@ void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restric
     }
 }
 
+// Batched 3D grids are narrow (tens to a few hundred columns): 2 columns per lane
+// (128-column strips) keep most lanes busy where 4 (256 columns) would idle half.
 template <typename S> struct Lanes3Db;
-template <> struct Lanes3Db<float> { static constexpr int V = 4; };
-template <> struct Lanes3Db<double> { static constexpr int V = 2; };
+template <> struct Lanes3Db<float> { static constexpr int V = 2; };
+template <> struct Lanes3Db<double> { static constexpr int V = 1; };
 constexpr int kNY3 = 4;   // rows per lane (radius 1)
 constexpr int kPF3 = 1;
 

@@ -35,7 +35,7 @@ def test_batch_mixed_2d_3d(storage, tol):
     items, refs = [], []
     for k, (h, w, shape, r) in enumerate(specs2d):
         src, x = _mk2d(h, w, 100 + k, pad=(k % 3) * 4, dtype=storage)
-        dst = torch.full_like(src, float("nan"))
+        dst, _ = _mk2d(h, w, 0, pad=(k % 3) * 4, dtype=storage)  # same row pitch as src
         f = wl.normalised_taps(shape, r, 7 + k)
         items.append(dict(ndim=2, src=src, dst=dst, coeffs=f,
                           shape=L.STAR if shape == "star" else L.BOX, radius=r))
