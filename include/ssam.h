@@ -61,8 +61,12 @@ typedef enum {
                                     (reference 3D gold, tmp3d7pt-v2.cu:284-296) */
 } ssam_boundary;
 
-#define SSAM_FLAG_COMPENSATED 1u /* compensated (Neumaier) combination of the
-                                    per-filter-row partial sums */
+#define SSAM_FLAG_COMPENSATED 1u /* Kahan-compensated accumulation: each filter
+                                    row is summed on its own and added to the
+                                    running sum with compensation.  BOX and DENSE
+                                    with steps == 1 (else SSAM_E_UNSUPPORTED);
+                                    FP64 storage accumulates in FP64 already and
+                                    is unchanged by it. */
 
 /* ---------------------------------------------------------------- descriptors */
 typedef struct {
@@ -71,7 +75,7 @@ typedef struct {
     int32_t fw, fh;       /* filter width / height; STAR and BOX: fw == fh == 2r+1 */
     int32_t steps;        /* time steps applied (>= 1); > 1 only for ZERO boundary */
     int32_t boundary;     /* ssam_boundary: ZERO, CLAMP or HOLD */
-    uint32_t flags;       /* SSAM_FLAG_* */
+    uint32_t flags;       /* SSAM_FLAG_* (other bits: SSAM_E_INVALID) */
     const double* coeffs; /* HOST memory, fh*fw taps row-major: w[m*fw + n] weights
                              in(x + n - fw/2, y + m - fh/2) (correlation, no flip).
                              STAR reads only the centre row and column. */
@@ -191,12 +195,15 @@ int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t n
 /* ---------------------------------------------------------------- batches
  * Many independent grids of uneven sizes and mixed shapes.  2D items of one
  * storage type whose shape is a 5-point star or a 3x3 / 5x5 / 11x11 box run as
- * ONE persistent launch whose waves take tiles from a device work queue (longest
- * tiles first); 3D items (7-point star or 19-point face+edge, F32/F64) run one
- * persistent launch per sweep; any other item gets its own launch.  The first
- * ssam_batch_run captures the whole sequence into a hipGraph on a private stream
- * and every run replays it on `stream` (SSAM_BATCH_EAGER: launch directly).
- * Device pointers are bound at plan creation.  No counterpart in the reference. */
+ * ONE persistent launch; 3D items (7-point star or 19-point face+edge, F32/F64)
+ * run one persistent launch per sweep for each (storage, steps, shape); any other
+ * item gets its own launch.  Persistent launches take tiles from 8 device work
+ * queues, one per XCD (neighbouring tiles of a grid share that XCD's L2), and
+ * steal from the other queues when their own is empty.  The first ssam_batch_run
+ * captures the whole sequence into a hipGraph on a private stream and every run
+ * replays it on `stream` (SSAM_BATCH_EAGER: launch directly).  Device pointers
+ * are bound at plan creation; items and taps are copied.  No counterpart in the
+ * reference. */
 typedef struct {
     int32_t ndim;          /* 2 or 3 */
     int32_t storage;       /* ssam_dtype (3D: F32 or F64) */
@@ -219,6 +226,16 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
 int ssam_batch_run(ssam_batch_plan plan, void* stream, uint32_t flags);
 int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles_3d,
                          int64_t* point_updates);
+/* Autotune the plan's tiling on its own buffers (dst and work are overwritten;
+ * the stream is synchronised): 2D tile height per shape (32..256 rows) and the 3D
+ * tile shape (columns and rows per lane, prefetch depth) and z-segment length,
+ * each timed over eager runs; the fastest is kept and the next ssam_batch_run
+ * captures a new graph. */
+int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
+/* The plan's tiling: 2D tile height per shape (4 entries: 5-point star, 3x3,
+ * 5x5, 11x11 box), 3D tile-shape index and z-segment length (any may be NULL). */
+int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* variant_3d,
+                           int32_t* zseg_3d);
 void ssam_batch_plan_destroy(ssam_batch_plan plan);
 
 /* ---------------------------------------------------------------- autotuner

@@ -233,6 +233,7 @@ ORC_API void orc_err_f64(const double* got, const double* ref, const double* sca
 #pragma omp parallel for reduction(max : mx, mrel) reduction(+ : s2, s1, match)
     for (int64_t i = 0; i < n; i++) {
         double d = fabs(got[i] - ref[i]);
+        if (d != d) d = INFINITY; /* an unwritten (NaN) or NaN result is never a match */
         if (d > mx) mx = d;
         s2 += d * d;
         s1 += d;

@@ -288,6 +288,20 @@ class BatchPlan:
         check(L.lib().ssam_batch_run(self._h, _stream(stream), L.BATCH_EAGER if eager else 0),
               "ssam_batch_run")
 
+    def tune(self, stream=None) -> dict:
+        """Autotune the tiling on the plan's buffers (ssam_batch_plan_tune; dst and
+        work are overwritten) and return the chosen tiling."""
+        check(L.lib().ssam_batch_plan_tune(self._h, _stream(stream)), "ssam_batch_plan_tune")
+        return self.tiling()
+
+    def tiling(self) -> dict:
+        seg = (ctypes.c_int32 * 4)()
+        var, zseg = ctypes.c_int32(), ctypes.c_int32()
+        check(L.lib().ssam_batch_plan_tiling(self._h, seg, ctypes.byref(var), ctypes.byref(zseg)),
+              "ssam_batch_plan_tiling")
+        return dict(seg_rows_2d=dict(zip(("star1", "box1", "box2", "box5"), list(seg))),
+                    variant_3d=var.value, zseg_3d=zseg.value)
+
     def info(self) -> tuple[int, int, int]:
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(L.lib().ssam_batch_plan_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),

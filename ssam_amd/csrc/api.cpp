@@ -116,6 +116,9 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
         return SSAM_E_INVALID;
     if (d->boundary != SSAM_BOUNDARY_ZERO && d->steps != 1) return SSAM_E_INVALID;
     if (d->fw < 1 || d->fh < 1) return SSAM_E_INVALID;
+    if (d->flags & ~SSAM_FLAG_COMPENSATED) return SSAM_E_INVALID;
+    const bool comp = (d->flags & SSAM_FLAG_COMPENSATED) != 0;
+    if (comp && (d->shape == SSAM_SHAPE_STAR || d->steps != 1)) return SSAM_E_UNSUPPORTED;
 
     // Embed the filter in a (2R+1)^2 box of taps centred like the reference's
     // correlation (offset n - fw/2, m - fh/2; common.h:196-197).
@@ -149,7 +152,8 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
                     if (m != R && n != R) taps[static_cast<size_t>(m) * K + n] = 0.0;
         }
     }
-    const int kshape = shape == SSAM_SHAPE_STAR ? 0 : 1;  // DENSE runs as a box
+    // DENSE runs as a box; compensated boxes are their own kernel instance
+    const int kshape = shape == SSAM_SHAPE_STAR ? kStar : (comp ? kBoxKahan : kBox);
 
     const int64_t bytes = ((height - 1) * pitch + width) * es;
     if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
@@ -563,7 +567,9 @@ int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst
     if (rc != SSAM_OK) return rc;
     int R = d->fw / 2 > d->fh / 2 ? d->fw / 2 : d->fh / 2;
     if (R < 1) R = 1;
-    const int kshape = d->shape == SSAM_SHAPE_STAR ? 0 : 1;
+    const int kshape = d->shape == SSAM_SHAPE_STAR
+                           ? kStar
+                           : ((d->flags & SSAM_FLAG_COMPENSATED) ? kBoxKahan : kBox);
     const int fmax = max_fused(d->storage, d->shape, R);
     const int wps_c[] = {2, 4, 8, 12};
     std::vector<int> tch_c;

@@ -1,23 +1,30 @@
 // batch.hip -- many grids of uneven sizes and mixed stencil shapes in one
-// persistent launch fed by a tile work queue (2D), multi-step 3D sweeps in one
+// persistent launch fed by tile work queues (2D), multi-step 3D sweeps in one
 // persistent launch per step, and the whole sequence captured in a hipGraph.
 //
 // No counterpart in the reference (one grid, one launch per harness call).
 //
 //  * Plan: every 2D item is cut into tiles (column strip x segment of rows) of the
-//    same geometry the single-grid launch uses; all tiles of one storage type go
-//    into one table, ordered by estimated cost (longest first) so the queue
-//    drains evenly.  3D items likewise (x-strip x y-block x z-segment).
-//  * Kernel: a fixed grid of waves; each wave takes the next tile index from a
-//    device counter (one vector atomic from lane 0), reads its item (scalar
-//    loads), and runs the same register-cache stream as the single-grid kernel for
-//    that item's shape; a wave leaves when the counter passes the table.
+//    same geometry the single-grid launch uses; 3D items into tiles of (x-strip x
+//    y-block x z-segment).  Each item's tiles stay in memory order (neighbouring
+//    tiles share halo rows) and are cut into runs; the runs are packed into 8
+//    queues of equal estimated cost, one per XCD, with the most expensive tiles of
+//    each queue first.
+//  * Kernel: a fixed grid of as many waves as the device holds at once; a wave
+//    reads its XCD id and takes the next tile of that XCD's queue (one vector
+//    atomic from lane 0), so neighbouring tiles run under one L2 and share their
+//    halo rows there; when its queue is empty it takes tiles from the other queues
+//    in turn (the result never depends on which XCD runs what), and leaves when all
+//    are empty.
 //  * Graph: counter reset (a fill kernel, no memset), the 2D launches, the per-step
 //    3D launches and any per-item fallback launches are captured once on a private
 //    stream and replayed on the caller's stream.
+//  * Tuner: ssam_batch_plan_tune times candidate tile heights per 2D shape and 3D
+//    tile shapes on the plan's own buffers and keeps the fastest.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -25,14 +32,15 @@
 #include "ssam_internal.hpp"
 #include "stencil2d_dispatch.hpp"
 #include "stencil3d.hpp"
+#include "stencil3d_hl.hpp"
 
 namespace ssam {
 
 // ------------------------------------------------------------------ 2D menu
-// Shapes the persistent 2D kernel runs in one launch (BASELINE-style mixes of
-// 5-point stars and 3x3 / 5x5 / 11x11 boxes); other shapes fall back to their own
-// launch.
+// Shapes the persistent 2D kernel runs in one launch (mixes of 5-point stars and
+// 3x3 / 5x5 / 11x11 boxes); other shapes fall back to their own launch.
 enum Kind2D : int { k2Star1 = 0, k2Box1 = 1, k2Box2 = 2, k2Box5 = 3, k2None = -1 };
+constexpr int kNumKind2D = 4;
 
 inline int kind2d(int shape, int r) {
     if (shape == kStar && r == 1) return k2Star1;
@@ -61,13 +69,23 @@ struct BatchItem3D {
     const void* src;
     void* dst;
     void* work;
-    Params3D p;
-    int kind;      // 0: 7-point star, 1: 19-point face+edge
+    Grid3D g;
+    int nstrips, nyb;
+    int vec;       // nx % V == 0 and every buffer aligned to V elements
     int steps;
     int taps_off;
 };
 
 constexpr int kBatchWavesPerBlock = 4;
+constexpr int kQueues = 8;  // one per XCD
+
+// First tile index of each queue (queue q holds tiles [beg[q], beg[q+1])).
+struct QueueSet { int beg[kQueues + 1]; };
+
+// Physical XCD of this wave (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int xcd_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kQueues - 1);
+}
 
 __device__ __forceinline__ int next_tile(int* counter) {
     int t = 0;
@@ -75,13 +93,28 @@ __device__ __forceinline__ int next_tile(int* counter) {
     return __builtin_amdgcn_readfirstlane(t);
 }
 
+// Run body(tile) for tiles of this XCD's queue, then of the others in turn,
+// until every queue is empty.  counters: one per queue, zero at launch.
+template <typename F>
+__device__ __forceinline__ void drain_queues(const QueueSet& qs, int* __restrict__ counters, F&& body) {
+    const int x = xcd_id();
+    for (int k = 0; k < kQueues; ++k) {
+        const int q = (x + k) & (kQueues - 1);
+        const int beg = qs.beg[q], end = qs.beg[q + 1];
+        if (beg >= end) continue;
+        for (;;) {
+            const int t = beg + next_tile(counters + q);
+            if (t >= end) break;
+            body(t);
+        }
+    }
+}
+
 template <typename S>
 __global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(2)))
-void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restrict__ tiles, int ntiles,
-                    int* __restrict__ counter, const typename Storage<S>::acc* __restrict__ taps) {
-    for (;;) {
-        const int t = next_tile(counter);
-        if (t >= ntiles) break;
+void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
+                    int* __restrict__ counters, const typename Storage<S>::acc* __restrict__ taps) {
+    drain_queues(qs, counters, [&](int t) {
         const int4 e = tiles[t];
         const int it = __builtin_amdgcn_readfirstlane(e.x);
         const int strip = __builtin_amdgcn_readfirstlane(e.y);
@@ -96,45 +129,47 @@ void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restric
             case k2Box2: Inst2D<S, 2, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
             default: Inst2D<S, 5, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
         }
-    }
+    });
 }
 
-// Batched 3D grids are narrow (tens to a few hundred columns): 2 columns per lane
-// (128-column strips) keep most lanes busy where 4 (256 columns) would idle half.
-template <typename S> struct Lanes3Db;
-template <> struct Lanes3Db<float> { static constexpr int V = 2; };
-template <> struct Lanes3Db<double> { static constexpr int V = 1; };
-constexpr int kNY3 = 4;   // rows per lane (radius 1)
-constexpr int kPF3 = 1;
+// Batched 3D tile shapes: (columns per lane, rows per lane, planes prefetched).
+// Narrow grids favour few columns per lane (62*V-column strips waste little of a
+// 64..256-column row); more rows per lane cut the y-halo re-reads.
+struct Var3D { int v, ny, pf; };
+constexpr Var3D kVar3D[] = {{1, 8, 2}, {1, 4, 3}, {2, 4, 2}, {2, 8, 1},
+                            {2, 8, 2}, {4, 4, 1}, {4, 4, 2}, {4, 8, 1}};
+constexpr int kNumVar3D = 8;
+constexpr int kDefaultVar3D = 3;
+constexpr int kDefaultZseg3D = 16;
+constexpr int kDefaultSeg2D = 64;
 
-template <typename S>
-__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(2)))
-void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restrict__ tiles, int ntiles,
-                    int* __restrict__ counter, const S* __restrict__ taps, int step) {
-    constexpr int V = Lanes3Db<S>::V;
-    for (;;) {
-        const int t = next_tile(counter);
-        if (t >= ntiles) break;
+template <typename S, int VAR> struct Var3DT {
+    // FP64 runs one column per lane (a plane row of NY+2 doubles per lane)
+    static constexpr int V = sizeof(S) == 8 ? 1 : kVar3D[VAR].v;
+    static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
+};
+
+// tile.x item, .y strip, .z y-block, .w first plane (low 20 bits) | planes << 20
+template <typename S, int VAR, int SHAPE>
+__global__ __launch_bounds__(kWave * kBatchWavesPerBlock)
+void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
+                    int* __restrict__ counters, const S* __restrict__ taps, int step) {
+    using T = Var3DT<S, VAR>;
+    drain_queues(qs, counters, [&](int t) {
         const int4 e = tiles[t];
         const int it = __builtin_amdgcn_readfirstlane(e.x);
         const int strip = __builtin_amdgcn_readfirstlane(e.y);
         const int yb = __builtin_amdgcn_readfirstlane(e.z);
-        const int zs = __builtin_amdgcn_readfirstlane(e.w);
+        const int zw = __builtin_amdgcn_readfirstlane(e.w);
+        const int zs = zw & 0xFFFFF, ze = zs + (zw >> 20);
         const BatchItem3D& b = items[it];
         // ping-pong so the last sweep writes dst (same rule as ssam_stencil3d)
         const void* in = step == 0 ? b.src : (((b.steps - step) % 2 == 0) ? b.dst : b.work);
         void* out = ((b.steps - 1 - step) % 2 == 0) ? b.dst : b.work;
-        const int ze = min(b.p.nz, zs + b.p.zseg);
-        const S* w = taps + b.taps_off;
-        if (b.kind == 0)
-            (void)Stencil3D<S, k3Star, 1, V, kNY3, kPF3>::run_tile(static_cast<const S*>(in),
-                                                                   static_cast<S*>(out), b.p, w,
-                                                                   strip, yb, zs, ze);
-        else
-            (void)Stencil3D<S, k3FaceEdge, 1, V, kNY3, kPF3>::run_tile(static_cast<const S*>(in),
-                                                                       static_cast<S*>(out), b.p,
-                                                                       w, strip, yb, zs, ze);
-    }
+        Stencil3DHL<S, SHAPE, T::V, T::NY, T::PF>::run_tile(static_cast<const S*>(in),
+                                                             static_cast<S*>(out), b.g, b.vec != 0,
+                                                             taps + b.taps_off, strip, yb, zs, ze);
+    });
 }
 
 __global__ void reset_counters_kernel(int* __restrict__ c, int n) {
@@ -149,28 +184,36 @@ using namespace ssam;
 // ------------------------------------------------------------------ plan
 struct ssam_batch_plan_s {
     int device = 0;
-    struct Group {  // one persistent launch: items of one storage type
+    struct Group {  // one persistent launch (2D) or one launch per step (3D)
         int ndim = 2;
         int storage = 0;
+        std::vector<int> ids;  // indices into items
         void* d_items = nullptr;
         void* d_tiles = nullptr;
         void* d_taps = nullptr;
         int ntiles = 0;
-        int steps = 1;  // 3D
+        QueueSet qs{};
+        int steps = 1;    // 3D
+        int kind = 0;     // 3D stencil: 0 7-point star, 1 19-point face+edge
+        int blocks = 1;   // persistent grid: resident workgroups of its kernel
     };
+    std::vector<ssam_batch_item> items;       // copies; coeffs point into taps_host
+    std::vector<std::vector<double>> taps_host;
     std::vector<Group> groups;
-    std::vector<ssam_batch_item> fallback;
-    std::vector<std::vector<double>> fallback_taps;
+    std::vector<int> fallback;                // indices into items
+    int seg2d[kNumKind2D] = {kDefaultSeg2D, kDefaultSeg2D, kDefaultSeg2D, kDefaultSeg2D};
+    int var3d = kDefaultVar3D, zseg3d = kDefaultZseg3D;
     int ncounters = 0;
     void* d_counters = nullptr;
     int64_t tiles2d = 0, tiles3d = 0, point_updates = 0;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int persistent_blocks = 1024;
 };
 
 namespace {
+
+using Plan = ssam_batch_plan_s;
 
 int elem_size_b(int dtype) {
     switch (dtype) {
@@ -182,9 +225,119 @@ int elem_size_b(int dtype) {
     }
 }
 
+int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return v && *v ? std::atoi(v) : dflt;
+}
+
+// Workgroups of `fn` the device holds at once (the persistent grid).
+int resident_blocks(const void* fn) {
+    const int forced = env_int("SSAM_BATCH_BLOCKS", 0);
+    if (forced > 0) return forced;
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kWave * kBatchWavesPerBlock, 0) !=
+            hipSuccess ||
+        per < 1)
+        per = 1;
+    return cus * per;
+}
+
+template <typename S>
+using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, const S*, int);
+
+template <typename S, int SHAPE> Batch3DFn<S> batch3d_fn_shape(int variant) {
+    switch (variant) {
+        case 0: return batch3d_kernel<S, 0, SHAPE>;
+        case 1: return batch3d_kernel<S, 1, SHAPE>;
+        case 2: return batch3d_kernel<S, 2, SHAPE>;
+        case 3: return batch3d_kernel<S, 3, SHAPE>;
+        case 4: return batch3d_kernel<S, 4, SHAPE>;
+        case 5: return batch3d_kernel<S, 5, SHAPE>;
+        case 6: return batch3d_kernel<S, 6, SHAPE>;
+        default: return batch3d_kernel<S, 7, SHAPE>;
+    }
+}
+template <typename S> const void* batch3d_fn(int variant, int kind) {
+    return reinterpret_cast<const void*>(kind ? batch3d_fn_shape<S, k3FaceEdge>(variant)
+                                              : batch3d_fn_shape<S, k3Star>(variant));
+}
+template <typename S> const void* batch2d_fn() { return reinterpret_cast<const void*>(batch2d_kernel<S>); }
+
+// One item's tiles in memory order with a per-tile cost estimate.
+struct ItemTiles {
+    std::vector<int4> tiles;
+    std::vector<double> cost;
+};
+
+// Pack items' tiles into kQueues queues: each item is cut into runs of
+// consecutive tiles (at most 1/32 of the total cost), runs are dealt to the least
+// loaded queue largest first, and inside a queue items go in decreasing per-tile
+// cost, each item's runs in memory order.
+void pack_queues(const std::vector<ItemTiles>& per_item, std::vector<int4>* tiles, QueueSet* qs) {
+    struct Run { int item; size_t b, e; double cost; };
+    double total = 0;
+    for (auto& it : per_item)
+        for (double c : it.cost) total += c;
+    const double cap = total / (kQueues * 4) > 0 ? total / (kQueues * 4) : 1.0;
+    std::vector<Run> runs;
+    for (int i = 0; i < static_cast<int>(per_item.size()); ++i) {
+        const ItemTiles& it = per_item[i];
+        size_t b = 0;
+        double acc = 0;
+        for (size_t k = 0; k < it.tiles.size(); ++k) {
+            acc += it.cost[k];
+            if (acc >= cap || k + 1 == it.tiles.size()) {
+                runs.push_back({i, b, k + 1, acc});
+                b = k + 1;
+                acc = 0;
+            }
+        }
+    }
+    std::vector<int> order(runs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return runs[a].cost > runs[b].cost; });
+    std::vector<std::vector<int>> bins(kQueues);
+    double load[kQueues] = {0};
+    for (int r : order) {
+        int q = 0;
+        for (int k = 1; k < kQueues; ++k)
+            if (load[k] < load[q]) q = k;
+        bins[q].push_back(r);
+        load[q] += runs[r].cost;
+    }
+    auto tile_cost = [&](int r) { return per_item[runs[r].item].cost[runs[r].b]; };
+    tiles->clear();
+    for (int q = 0; q < kQueues; ++q) {
+        std::vector<int>& bq = bins[q];
+        std::sort(bq.begin(), bq.end(), [&](int a, int b) {
+            const double ca = tile_cost(a), cb = tile_cost(b);
+            if (ca != cb) return ca > cb;
+            return runs[a].item != runs[b].item ? runs[a].item < runs[b].item : runs[a].b < runs[b].b;
+        });
+        qs->beg[q] = static_cast<int>(tiles->size());
+        for (int r : bq)
+            for (size_t k = runs[r].b; k < runs[r].e; ++k) tiles->push_back(per_item[runs[r].item].tiles[k]);
+    }
+    qs->beg[kQueues] = static_cast<int>(tiles->size());
+}
+
+template <typename T>
+int upload(void** dptr, const std::vector<T>& h) {
+    if (hipMalloc(dptr, std::max<size_t>(1, h.size()) * sizeof(T)) != hipSuccess) {
+        *dptr = nullptr;
+        return SSAM_E_HIP;
+    }
+    if (!h.empty() && hipMemcpy(*dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return SSAM_E_HIP;
+    return SSAM_OK;
+}
+
 template <typename S, int R, int SHAPE>
-void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p,
-                  std::vector<std::pair<double, int4>>* out) {
+void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p, ItemTiles* out) {
     using I = Inst2D<S, R, SHAPE>;
     p->width = static_cast<int>(it.nx);
     p->height = static_cast<int>(it.ny);
@@ -200,21 +353,29 @@ void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p,
     const double taps = SHAPE == kStar ? 4 * R + 1 : (2 * R + 1) * (2 * R + 1);
     for (int sg = 0; sg < nsegs; ++sg) {
         const int rows = std::min(seg_rows, p->height - sg * seg_rows);
-        for (int st = 0; st < p->nstrips; ++st)
-            out->push_back({static_cast<double>(rows + 2 * R) * (taps + 8.0), int4{idx, st, sg, 0}});
+        for (int st = 0; st < p->nstrips; ++st) {
+            out->tiles.push_back(int4{idx, st, sg, 0});
+            out->cost.push_back(static_cast<double>(rows + 2 * R) * (taps + 8.0));
+        }
     }
 }
 
+void free_group(Plan::Group& g) {
+    if (g.d_items) (void)hipFree(g.d_items);
+    if (g.d_tiles) (void)hipFree(g.d_tiles);
+    if (g.d_taps) (void)hipFree(g.d_taps);
+    g.d_items = g.d_tiles = g.d_taps = nullptr;
+}
+
+// (Re)build the device tables of a 2D group with the plan's tile heights.
 template <typename S>
-int build_group2d(ssam_batch_plan_s* P, const std::vector<int>& ids, const ssam_batch_item* items,
-                  int storage) {
+int build_group2d(Plan* P, Plan::Group& g) {
     using A = typename Storage<S>::acc;
-    std::vector<BatchItem2D> h(ids.size());
+    std::vector<BatchItem2D> h(g.ids.size());
     std::vector<A> taps;
-    std::vector<std::pair<double, int4>> tl;
-    const int seg_rows = 32;
-    for (size_t k = 0; k < ids.size(); ++k) {
-        const ssam_batch_item& it = items[ids[k]];
+    std::vector<ItemTiles> per(g.ids.size());
+    for (size_t k = 0; k < g.ids.size(); ++k) {
+        const ssam_batch_item& it = P->items[g.ids[k]];
         const int R = it.radius, K = 2 * R + 1;
         const int shape = it.shape == SSAM_SHAPE_STAR ? kStar : kBox;
         BatchItem2D& b = h[k];
@@ -229,220 +390,227 @@ int build_group2d(ssam_batch_plan_s* P, const std::vector<int>& ids, const ssam_
                 taps.push_back(static_cast<A>(v));
             }
         const int idx = static_cast<int>(k);
+        const int seg = P->seg2d[b.kind];
         switch (b.kind) {
-            case k2Star1: tiles_for_2d<S, 1, kStar>(it, idx, seg_rows, &b.p, &tl); break;
-            case k2Box1: tiles_for_2d<S, 1, kBox>(it, idx, seg_rows, &b.p, &tl); break;
-            case k2Box2: tiles_for_2d<S, 2, kBox>(it, idx, seg_rows, &b.p, &tl); break;
-            default: tiles_for_2d<S, 5, kBox>(it, idx, seg_rows, &b.p, &tl); break;
+            case k2Star1: tiles_for_2d<S, 1, kStar>(it, idx, seg, &b.p, &per[k]); break;
+            case k2Box1: tiles_for_2d<S, 1, kBox>(it, idx, seg, &b.p, &per[k]); break;
+            case k2Box2: tiles_for_2d<S, 2, kBox>(it, idx, seg, &b.p, &per[k]); break;
+            default: tiles_for_2d<S, 5, kBox>(it, idx, seg, &b.p, &per[k]); break;
         }
-        P->point_updates += it.nx * it.ny;
     }
-    // longest tiles first (LPT), stable for reproducible schedules
-    std::stable_sort(tl.begin(), tl.end(),
-                     [](const std::pair<double, int4>& a, const std::pair<double, int4>& b) {
-                         return a.first > b.first;
-                     });
-    std::vector<int4> tiles(tl.size());
-    for (size_t i = 0; i < tl.size(); ++i) tiles[i] = tl[i].second;
-    ssam_batch_plan_s::Group g;
-    g.ndim = 2;
-    g.storage = storage;
+    std::vector<int4> tiles;
+    pack_queues(per, &tiles, &g.qs);
+    free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
-    P->groups.push_back(g);  // owned by the plan from here on (freed on failure too)
-    ssam_batch_plan_s::Group& gg = P->groups.back();
-    if (hipMalloc(&gg.d_items, h.size() * sizeof(BatchItem2D)) != hipSuccess ||
-        hipMalloc(&gg.d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(int4)) != hipSuccess ||
-        hipMalloc(&gg.d_taps, std::max<size_t>(1, taps.size()) * sizeof(A)) != hipSuccess)
-        return SSAM_E_HIP;
-    g = gg;
-    if (hipMemcpy(g.d_items, h.data(), h.size() * sizeof(BatchItem2D), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        (tiles.size() && hipMemcpy(g.d_tiles, tiles.data(), tiles.size() * sizeof(int4),
-                                   hipMemcpyHostToDevice) != hipSuccess) ||
-        (taps.size() &&
-         hipMemcpy(g.d_taps, taps.data(), taps.size() * sizeof(A), hipMemcpyHostToDevice) != hipSuccess))
-        return SSAM_E_HIP;
-    P->tiles2d += g.ntiles;
-    return SSAM_OK;
+    g.blocks = resident_blocks(batch2d_fn<S>());
+    int rc = upload(&g.d_items, h);
+    if (rc == SSAM_OK) rc = upload(&g.d_tiles, tiles);
+    if (rc == SSAM_OK) rc = upload(&g.d_taps, taps);
+    return rc;
 }
 
+// (Re)build the device tables of a 3D group with the plan's tile shape.
 template <typename S>
-int build_group3d(ssam_batch_plan_s* P, const std::vector<int>& ids, const ssam_batch_item* items,
-                  int storage) {
-    constexpr int V = Lanes3Db<S>::V;
-    std::vector<BatchItem3D> h(ids.size());
+int build_group3d(Plan* P, Plan::Group& g) {
+    const int variant = P->var3d, zseg = P->zseg3d;
+    const int V = sizeof(S) == 8 ? 1 : kVar3D[variant].v, NY = kVar3D[variant].ny;
+    const int OW = (kWave - 2) * V;
+    std::vector<BatchItem3D> h(g.ids.size());
     std::vector<S> taps;
-    std::vector<std::pair<double, int4>> tl;
-    const int zseg = 16;
-    int steps = 0;
-    for (size_t k = 0; k < ids.size(); ++k) {
-        const ssam_batch_item& it = items[ids[k]];
+    std::vector<ItemTiles> per(g.ids.size());
+    for (size_t k = 0; k < g.ids.size(); ++k) {
+        const ssam_batch_item& it = P->items[g.ids[k]];
         BatchItem3D& b = h[k];
         b.src = it.src;
         b.dst = it.dst;
         b.work = it.work;
-        b.kind = it.shape == SSAM_SHAPE_FACE_EDGE ? 1 : 0;
         b.steps = it.steps;
-        if (steps == 0) steps = it.steps;
-        if (it.steps != steps) return SSAM_E_UNSUPPORTED;  // one step count per group
         b.taps_off = static_cast<int>(taps.size());
-        const int nt = b.kind ? 19 : 7;
+        const int nt = g.kind ? 19 : 7;
         for (int t = 0; t < nt; ++t) taps.push_back(static_cast<S>(it.coeffs[t]));
-        Params3D& p = b.p;
-        p.nx = static_cast<int>(it.nx);
-        p.ny = static_cast<int>(it.ny);
-        p.nz = static_cast<int>(it.nz);
-        p.sy = it.nx;
-        p.sz = it.nx * it.ny;
-        p.nstrips = static_cast<int>((it.nx + kWave * V - 1) / (kWave * V));
-        p.nyb = static_cast<int>((it.ny + kNY3 - 1) / kNY3);
-        p.zseg = static_cast<int>(std::min<int64_t>(zseg, it.nz));
-        const int nzs = (p.nz + p.zseg - 1) / p.zseg;
-        p.ntiles = p.nstrips * p.nyb * nzs;
-        p.nblocks = 0;
-        p.vec_ok = (it.nx % V == 0) && (reinterpret_cast<uintptr_t>(it.src) % 16 == 0) &&
-                   (reinterpret_cast<uintptr_t>(it.dst) % 16 == 0) &&
-                   (!it.work || reinterpret_cast<uintptr_t>(it.work) % 16 == 0);
-        p.partials = nullptr;
-        p.res_scale = 0.0;
-        for (int z = 0; z < nzs; ++z)
-            for (int y = 0; y < p.nyb; ++y)
-                for (int s = 0; s < p.nstrips; ++s) {
-                    const int w = std::min<int>(kWave * V, p.nx - s * kWave * V);
-                    const int planes = std::min(p.zseg, p.nz - z * p.zseg);
-                    tl.push_back({static_cast<double>(w) * (planes + 2) * nt,
-                                  int4{static_cast<int>(k), s, y, z * p.zseg}});
+        Grid3D& gr = b.g;
+        gr.nx = static_cast<int>(it.nx);
+        gr.ny = static_cast<int>(it.ny);
+        gr.nz = static_cast<int>(it.nz);
+        gr.sy = it.nx;
+        gr.sz = it.nx * it.ny;
+        b.nstrips = static_cast<int>((it.nx + OW - 1) / OW);
+        b.nyb = static_cast<int>((it.ny + NY - 1) / NY);
+        const uintptr_t al = static_cast<uintptr_t>(V) * sizeof(S);
+        b.vec = (it.nx % V == 0) && (reinterpret_cast<uintptr_t>(it.src) % al == 0) &&
+                (reinterpret_cast<uintptr_t>(it.dst) % al == 0) &&
+                (!it.work || reinterpret_cast<uintptr_t>(it.work) % al == 0);
+        const int zs_len = std::min<int>(zseg, gr.nz);
+        for (int z0 = 0; z0 < gr.nz; z0 += zs_len) {
+            const int planes = std::min(zs_len, gr.nz - z0);
+            for (int y = 0; y < b.nyb; ++y)
+                for (int s = 0; s < b.nstrips; ++s) {
+                    const int rows = std::min(NY, gr.ny - y * NY);
+                    per[k].tiles.push_back(int4{static_cast<int>(k), s, y, z0 | (planes << 20)});
+                    // rows loaded x planes streamed (+2 halo planes) x taps
+                    per[k].cost.push_back(static_cast<double>(rows + 2) * (planes + 2) * nt);
                 }
-        P->point_updates += it.nx * it.ny * it.nz * it.steps;
+        }
     }
-    std::stable_sort(tl.begin(), tl.end(),
-                     [](const std::pair<double, int4>& a, const std::pair<double, int4>& b) {
-                         return a.first > b.first;
-                     });
-    std::vector<int4> tiles(tl.size());
-    for (size_t i = 0; i < tl.size(); ++i) tiles[i] = tl[i].second;
-    ssam_batch_plan_s::Group g;
-    g.ndim = 3;
-    g.storage = storage;
-    g.steps = steps;
+    std::vector<int4> tiles;
+    pack_queues(per, &tiles, &g.qs);
+    free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
-    P->groups.push_back(g);  // owned by the plan from here on (freed on failure too)
-    ssam_batch_plan_s::Group& gg = P->groups.back();
-    if (hipMalloc(&gg.d_items, h.size() * sizeof(BatchItem3D)) != hipSuccess ||
-        hipMalloc(&gg.d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(int4)) != hipSuccess ||
-        hipMalloc(&gg.d_taps, std::max<size_t>(1, taps.size()) * sizeof(S)) != hipSuccess)
-        return SSAM_E_HIP;
-    g = gg;
-    if (hipMemcpy(g.d_items, h.data(), h.size() * sizeof(BatchItem3D), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        (tiles.size() && hipMemcpy(g.d_tiles, tiles.data(), tiles.size() * sizeof(int4),
-                                   hipMemcpyHostToDevice) != hipSuccess) ||
-        (taps.size() &&
-         hipMemcpy(g.d_taps, taps.data(), taps.size() * sizeof(S), hipMemcpyHostToDevice) != hipSuccess))
-        return SSAM_E_HIP;
-    P->tiles3d += static_cast<int64_t>(g.ntiles) * steps;
+    g.blocks = resident_blocks(batch3d_fn<S>(variant, g.kind));
+    int rc = upload(&g.d_items, h);
+    if (rc == SSAM_OK) rc = upload(&g.d_tiles, tiles);
+    if (rc == SSAM_OK) rc = upload(&g.d_taps, taps);
+    return rc;
+}
+
+int build_group(Plan* P, Plan::Group& g) {
+    if (g.ndim == 2) {
+        switch (g.storage) {
+            case SSAM_F32: return build_group2d<float>(P, g);
+            case SSAM_F64: return build_group2d<double>(P, g);
+            case SSAM_BF16: return build_group2d<bf16_t>(P, g);
+            default: return build_group2d<f16_t>(P, g);
+        }
+    }
+    return g.storage == SSAM_F64 ? build_group3d<double>(P, g) : build_group3d<float>(P, g);
+}
+
+int build_all(Plan* P) {
+    P->tiles2d = P->tiles3d = 0;
+    for (auto& g : P->groups) {
+        const int rc = build_group(P, g);
+        if (rc != SSAM_OK) return rc;
+        if (g.ndim == 2) P->tiles2d += g.ntiles;
+        else P->tiles3d += static_cast<int64_t>(g.ntiles) * g.steps;
+    }
     return SSAM_OK;
 }
 
-void free_plan(ssam_batch_plan_s* P) {
-    if (!P) return;
+void drop_graph(Plan* P) {
     if (P->exec) (void)hipGraphExecDestroy(P->exec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
+    P->exec = nullptr;
+    P->graph = nullptr;
+}
+
+void free_plan(Plan* P) {
+    if (!P) return;
+    drop_graph(P);
     if (P->cap) (void)hipStreamDestroy(P->cap);
-    for (auto& g : P->groups) {
-        if (g.d_items) (void)hipFree(g.d_items);
-        if (g.d_tiles) (void)hipFree(g.d_tiles);
-        if (g.d_taps) (void)hipFree(g.d_taps);
-    }
+    for (auto& g : P->groups) free_group(g);
     if (P->d_counters) (void)hipFree(P->d_counters);
     delete P;
 }
 
-// Every launch of one batch run, in order, on stream st.
-int enqueue(ssam_batch_plan_s* P, hipStream_t st) {
+constexpr unsigned kRun2D = 1, kRun3D = 2, kRunFallback = 4, kRunAll = 7;
+
+// The launches of one batch run, in order, on stream st (parts: kRun* mask).
+int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
     hipLaunchKernelGGL(reset_counters_kernel, dim3((P->ncounters + 255) / 256), dim3(256), 0, st,
                        static_cast<int*>(P->d_counters), P->ncounters);
     if (hipGetLastError() != hipSuccess) return SSAM_E_HIP;
-    int ctr = 0;
     int* counters = static_cast<int*>(P->d_counters);
+    int ctr = 0;  // counter sets (kQueues counters each): 2D groups, then 3D (step, group)
     for (auto& g : P->groups) {
         if (g.ndim != 2) continue;
-        const int blocks = std::max(1, std::min(P->persistent_blocks,
-                                                (g.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock));
+        int* cnt = counters + kQueues * ctr++;
+        if (!(parts & kRun2D)) continue;
+        const int blocks = std::max(1, std::min(g.blocks, (g.ntiles + kBatchWavesPerBlock - 1) /
+                                                              kBatchWavesPerBlock));
         const dim3 grid(blocks), block(kWave * kBatchWavesPerBlock);
         const auto* items = static_cast<const BatchItem2D*>(g.d_items);
         const auto* tiles = static_cast<const int4*>(g.d_tiles);
         switch (g.storage) {
             case SSAM_F32:
-                hipLaunchKernelGGL(batch2d_kernel<float>, grid, block, 0, st, items, tiles, g.ntiles,
-                                   counters + ctr, static_cast<const float*>(g.d_taps));
+                hipLaunchKernelGGL(batch2d_kernel<float>, grid, block, 0, st, items, tiles, g.qs, cnt,
+                                   static_cast<const float*>(g.d_taps));
                 break;
             case SSAM_F64:
-                hipLaunchKernelGGL(batch2d_kernel<double>, grid, block, 0, st, items, tiles, g.ntiles,
-                                   counters + ctr, static_cast<const double*>(g.d_taps));
+                hipLaunchKernelGGL(batch2d_kernel<double>, grid, block, 0, st, items, tiles, g.qs, cnt,
+                                   static_cast<const double*>(g.d_taps));
                 break;
             case SSAM_BF16:
-                hipLaunchKernelGGL(batch2d_kernel<bf16_t>, grid, block, 0, st, items, tiles, g.ntiles,
-                                   counters + ctr, static_cast<const float*>(g.d_taps));
+                hipLaunchKernelGGL(batch2d_kernel<bf16_t>, grid, block, 0, st, items, tiles, g.qs, cnt,
+                                   static_cast<const float*>(g.d_taps));
                 break;
             default:
-                hipLaunchKernelGGL(batch2d_kernel<f16_t>, grid, block, 0, st, items, tiles, g.ntiles,
-                                   counters + ctr, static_cast<const float*>(g.d_taps));
+                hipLaunchKernelGGL(batch2d_kernel<f16_t>, grid, block, 0, st, items, tiles, g.qs, cnt,
+                                   static_cast<const float*>(g.d_taps));
                 break;
         }
         if (hipGetLastError() != hipSuccess) return SSAM_E_HIP;
-        ++ctr;
     }
-    for (size_t i = 0; i < P->fallback.size(); ++i) {
-        const ssam_batch_item& it = P->fallback[i];
-        int rc;
-        if (it.ndim == 2) {
-            ssam_stencil2d_desc d{};
-            d.storage = it.storage;
-            d.shape = it.shape;
-            d.fw = d.fh = 2 * it.radius + 1;
-            d.steps = it.steps;
-            d.boundary = SSAM_BOUNDARY_ZERO;
-            d.coeffs = P->fallback_taps[i].data();
-            rc = ssam_stencil2d(&d, it.src, it.dst, it.nx, it.ny, it.pitch, it.work, st);
-        } else {
-            ssam_stencil3d_desc d{};
-            d.storage = it.storage;
-            d.shape = it.shape;
-            d.radius = it.radius;
-            d.steps = it.steps;
-            d.boundary = SSAM_BOUNDARY_DIRICHLET0;
-            d.coeffs = P->fallback_taps[i].data();
-            rc = ssam_stencil3d(&d, it.src, it.dst, it.nx, it.ny, it.nz, it.work, st);
+    if (parts & kRunFallback) {
+        for (int i : P->fallback) {
+            const ssam_batch_item& it = P->items[i];
+            int rc;
+            if (it.ndim == 2) {
+                ssam_stencil2d_desc d{};
+                d.storage = it.storage;
+                d.shape = it.shape;
+                d.fw = d.fh = 2 * it.radius + 1;
+                d.steps = it.steps;
+                d.boundary = SSAM_BOUNDARY_ZERO;
+                d.coeffs = it.coeffs;
+                rc = ssam_stencil2d(&d, it.src, it.dst, it.nx, it.ny, it.pitch, it.work, st);
+            } else {
+                ssam_stencil3d_desc d{};
+                d.storage = it.storage;
+                d.shape = it.shape;
+                d.radius = it.radius;
+                d.steps = it.steps;
+                d.boundary = SSAM_BOUNDARY_DIRICHLET0;
+                d.coeffs = it.coeffs;
+                rc = ssam_stencil3d(&d, it.src, it.dst, it.nx, it.ny, it.nz, it.work, st);
+            }
+            if (rc != SSAM_OK) return rc;
         }
-        if (rc != SSAM_OK) return rc;
     }
     int maxsteps = 0;
     for (auto& g : P->groups)
         if (g.ndim == 3) maxsteps = std::max(maxsteps, g.steps);
     for (int s = 0; s < maxsteps; ++s) {
-        int c3 = ctr;
         for (auto& g : P->groups) {
             if (g.ndim != 3) continue;
-            if (s < g.steps) {
-                const int blocks = std::max(1, std::min(P->persistent_blocks,
-                                                        (g.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock));
-                const dim3 grid(blocks), block(kWave * kBatchWavesPerBlock);
-                const auto* items = static_cast<const BatchItem3D*>(g.d_items);
-                const auto* tiles = static_cast<const int4*>(g.d_tiles);
-                if (g.storage == SSAM_F64)
-                    hipLaunchKernelGGL(batch3d_kernel<double>, grid, block, 0, st, items, tiles,
-                                       g.ntiles, counters + c3, static_cast<const double*>(g.d_taps), s);
-                else
-                    hipLaunchKernelGGL(batch3d_kernel<float>, grid, block, 0, st, items, tiles,
-                                       g.ntiles, counters + c3, static_cast<const float*>(g.d_taps), s);
-                if (hipGetLastError() != hipSuccess) return SSAM_E_HIP;
-            }
-            ++c3;
+            int* cnt = counters + kQueues * ctr++;
+            if (!(parts & kRun3D) || s >= g.steps) continue;
+            const int blocks = std::max(1, std::min(g.blocks, (g.ntiles + kBatchWavesPerBlock - 1) /
+                                                                  kBatchWavesPerBlock));
+            const BatchItem3D* items = static_cast<const BatchItem3D*>(g.d_items);
+            const int4* tiles = static_cast<const int4*>(g.d_tiles);
+            QueueSet qs = g.qs;
+            int step = s;
+            const void* taps = g.d_taps;
+            void* args[] = {&items, &tiles, &qs, &cnt, &taps, &step};
+            const void* fn = g.storage == SSAM_F64 ? batch3d_fn<double>(P->var3d, g.kind)
+                                                   : batch3d_fn<float>(P->var3d, g.kind);
+            if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
+                    hipSuccess ||
+                hipGetLastError() != hipSuccess)
+                return SSAM_E_HIP;
         }
-        ctr = c3;
     }
     return SSAM_OK;
+}
+
+// Mean time of `reps` eager runs of the given parts (after one warm-up run).
+int time_parts(Plan* P, hipStream_t st, unsigned parts, int reps, float* ms) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        if (a) (void)hipEventDestroy(a);
+        return SSAM_E_HIP;
+    }
+    int rc = enqueue(P, st, parts);
+    if (rc == SSAM_OK && hipEventRecord(a, st) != hipSuccess) rc = SSAM_E_HIP;
+    for (int r = 0; r < reps && rc == SSAM_OK; ++r) rc = enqueue(P, st, parts);
+    if (rc == SSAM_OK && (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess))
+        rc = SSAM_E_HIP;
+    float t = 0.f;
+    if (rc == SSAM_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = SSAM_E_HIP;
+    *ms = t / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
 }
 
 }  // namespace
@@ -453,7 +621,9 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
     if (!items || n < 1 || !out) return SSAM_E_INVALID;
     *out = nullptr;
     // validate and group
-    std::vector<int> g2[4], g3[2];
+    std::vector<int> g2[4];
+    std::vector<std::vector<int>> g3;  // one group per (storage, steps, shape)
+    std::vector<int64_t> g3key;
     std::vector<int> fb;
     for (int i = 0; i < n; ++i) {
         const ssam_batch_item& it = items[i];
@@ -479,51 +649,68 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
             const char* a = static_cast<const char*>(it.src);
             const char* b = static_cast<const char*>(it.dst);
             if (a < b + bytes && b < a + bytes) return SSAM_E_INVALID;
-            if (it.radius == 1 && (it.shape == SSAM_SHAPE_STAR || it.shape == SSAM_SHAPE_FACE_EDGE))
-                g3[it.storage == SSAM_F64 ? 1 : 0].push_back(i);
-            else
+            if (it.radius == 1 && (it.shape == SSAM_SHAPE_STAR || it.shape == SSAM_SHAPE_FACE_EDGE)) {
+                // tile fields: first plane in 20 bits, plane count in 11
+                if (it.nz >= (1 << 20) || it.nx > (1 << 30) || it.ny > (1 << 30))
+                    return SSAM_E_UNSUPPORTED;
+                const int64_t key = (static_cast<int64_t>(it.steps) * 2 +
+                                     (it.shape == SSAM_SHAPE_FACE_EDGE ? 1 : 0)) * 2 +
+                                    (it.storage == SSAM_F64 ? 1 : 0);
+                auto pos = std::find(g3key.begin(), g3key.end(), key);
+                if (pos == g3key.end()) {
+                    g3key.push_back(key);
+                    g3.push_back({i});
+                } else {
+                    g3[pos - g3key.begin()].push_back(i);
+                }
+            } else {
                 fb.push_back(i);
+            }
         } else {
             return SSAM_E_INVALID;
         }
     }
-    auto* P = new (std::nothrow) ssam_batch_plan_s();
+    auto* P = new (std::nothrow) Plan();
     if (!P) return SSAM_E_NOMEM;
     (void)hipGetDevice(&P->device);
-    int rc = SSAM_OK;
-    if (!g2[SSAM_F32].empty()) rc = build_group2d<float>(P, g2[SSAM_F32], items, SSAM_F32);
-    if (rc == SSAM_OK && !g2[SSAM_F64].empty()) rc = build_group2d<double>(P, g2[SSAM_F64], items, SSAM_F64);
-    if (rc == SSAM_OK && !g2[SSAM_BF16].empty()) rc = build_group2d<bf16_t>(P, g2[SSAM_BF16], items, SSAM_BF16);
-    if (rc == SSAM_OK && !g2[SSAM_F16].empty()) rc = build_group2d<f16_t>(P, g2[SSAM_F16], items, SSAM_F16);
-    // 3D groups: split by step count inside build_group3d (one count per group)
-    for (int s = 0; s < 2 && rc == SSAM_OK; ++s) {
-        if (g3[s].empty()) continue;
-        std::vector<std::vector<int>> bysteps;
-        std::vector<int> keys;
-        for (int i : g3[s]) {
-            auto pos = std::find(keys.begin(), keys.end(), items[i].steps);
-            if (pos == keys.end()) {
-                keys.push_back(items[i].steps);
-                bysteps.push_back({i});
-            } else {
-                bysteps[pos - keys.begin()].push_back(i);
-            }
-        }
-        for (auto& ids : bysteps) {
-            rc = s ? build_group3d<double>(P, ids, items, SSAM_F64) : build_group3d<float>(P, ids, items, SSAM_F32);
-            if (rc != SSAM_OK) break;
-        }
-    }
-    for (int i : fb) {
-        const ssam_batch_item& it = items[i];
-        P->fallback.push_back(it);
+    // private copies of the items and their taps (the caller's arrays may go away)
+    P->items.assign(items, items + n);
+    P->taps_host.resize(n);
+    for (int i = 0; i < n; ++i) {
+        ssam_batch_item& it = P->items[i];
         const int nt = it.ndim == 2 ? (2 * it.radius + 1) * (2 * it.radius + 1)
                                     : (it.shape == SSAM_SHAPE_FACE_EDGE ? 19 : 6 * it.radius + 1);
-        P->fallback_taps.emplace_back(it.coeffs, it.coeffs + nt);
+        P->taps_host[i].assign(it.coeffs, it.coeffs + nt);
+        it.coeffs = P->taps_host[i].data();
         P->point_updates += it.nx * it.ny * it.nz * it.steps;
     }
+    P->fallback = fb;
+    const int v = env_int("SSAM_BATCH3D_VARIANT", kDefaultVar3D);
+    P->var3d = v >= 0 && v < kNumVar3D ? v : kDefaultVar3D;
+    P->zseg3d = std::max(1, std::min(env_int("SSAM_BATCH3D_ZSEG", kDefaultZseg3D), 2047));
+    const int seg = env_int("SSAM_BATCH2D_SEG", kDefaultSeg2D);
+    for (int k = 0; k < kNumKind2D; ++k) P->seg2d[k] = seg >= 8 ? seg : kDefaultSeg2D;
+    for (int s = 0; s < 4; ++s) {
+        if (g2[s].empty()) continue;
+        Plan::Group g;
+        g.ndim = 2;
+        g.storage = s;
+        g.ids = g2[s];
+        P->groups.push_back(g);
+    }
+    for (size_t k = 0; k < g3.size(); ++k) {
+        Plan::Group g;
+        g.ndim = 3;
+        const ssam_batch_item& it = items[g3[k][0]];
+        g.storage = it.storage;
+        g.steps = it.steps;
+        g.kind = it.shape == SSAM_SHAPE_FACE_EDGE ? 1 : 0;
+        g.ids = g3[k];
+        P->groups.push_back(g);
+    }
+    int rc = build_all(P);
     if (rc == SSAM_OK) {
-        // one counter per 2D launch and per (3D group, step) slot of enqueue()
+        // kQueues counters per 2D launch and per (3D group, step) slot of enqueue()
         int n2 = 0, n3 = 0, maxsteps = 0;
         for (auto& g : P->groups) {
             if (g.ndim == 2) ++n2;
@@ -532,7 +719,7 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
                 maxsteps = std::max(maxsteps, g.steps);
             }
         }
-        P->ncounters = std::max(1, n2 + n3 * maxsteps);
+        P->ncounters = std::max(1, kQueues * (n2 + n3 * maxsteps));
         if (hipMalloc(&P->d_counters, P->ncounters * sizeof(int)) != hipSuccess) rc = SSAM_E_HIP;
     }
     if (rc != SSAM_OK) {
@@ -568,6 +755,82 @@ int ssam_batch_run(ssam_batch_plan plan, void* stream, uint32_t flags) {
         }
     }
     return hipGraphLaunch(plan->exec, st) == hipSuccess ? SSAM_OK : SSAM_E_HIP;
+}
+
+int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
+    if (!plan) return SSAM_E_INVALID;
+    Plan* P = plan;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    drop_graph(P);  // the tables change; the next run captures anew
+    const int reps = 2;
+    bool has2 = false, has3 = false, kinds[kNumKind2D] = {false, false, false, false};
+    for (auto& g : P->groups) {
+        if (g.ndim == 3) {
+            has3 = true;
+        } else {
+            has2 = true;
+            for (int i : g.ids) {
+                const ssam_batch_item& it = P->items[i];
+                kinds[kind2d(it.shape == SSAM_SHAPE_STAR ? kStar : kBox, it.radius)] = true;
+            }
+        }
+    }
+    float ms = 0.f;
+    int rc = SSAM_OK;
+    // 2D: tile height per shape, one shape at a time (the others held)
+    const int segs[] = {32, 64, 128, 256};
+    for (int k = 0; k < kNumKind2D && has2 && rc == SSAM_OK; ++k) {
+        if (!kinds[k]) continue;
+        int best = P->seg2d[k];
+        float best_ms = 1e30f;
+        for (int s : segs) {
+            P->seg2d[k] = s;
+            rc = build_all(P);
+            if (rc == SSAM_OK) rc = time_parts(P, st, kRun2D, reps, &ms);
+            if (rc != SSAM_OK) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = s;
+            }
+        }
+        P->seg2d[k] = best;
+    }
+    // 3D: tile shape x z-segment length
+    if (has3 && rc == SSAM_OK) {
+        const int vars[] = {0, 2, 3, 4};
+        const int zsegs[] = {16, 32};
+        int bv = P->var3d, bz = P->zseg3d;
+        float best_ms = 1e30f;
+        for (int v : vars) {
+            for (int z : zsegs) {
+                P->var3d = v;
+                P->zseg3d = z;
+                rc = build_all(P);
+                if (rc == SSAM_OK) rc = time_parts(P, st, kRun3D, reps, &ms);
+                if (rc != SSAM_OK) break;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    bv = v;
+                    bz = z;
+                }
+            }
+            if (rc != SSAM_OK) break;
+        }
+        P->var3d = bv;
+        P->zseg3d = bz;
+    }
+    const int rc2 = build_all(P);
+    return rc != SSAM_OK ? rc : rc2;
+}
+
+int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* variant_3d,
+                           int32_t* zseg_3d) {
+    if (!plan) return SSAM_E_INVALID;
+    if (seg_rows_2d)
+        for (int k = 0; k < kNumKind2D; ++k) seg_rows_2d[k] = plan->seg2d[k];
+    if (variant_3d) *variant_3d = plan->var3d;
+    if (zseg_3d) *zseg_3d = plan->zseg3d;
+    return SSAM_OK;
 }
 
 int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles_3d,

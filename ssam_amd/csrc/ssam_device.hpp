@@ -84,6 +84,29 @@ template <typename A> __device__ __forceinline__ A from_right(A v) { return dpp_
 // value of lane i-1 (data moves toward higher lanes)
 template <typename A> __device__ __forceinline__ A from_left(A v) { return dpp_move<0x138>(v); }
 
+// ---------------------------------------------------------------- global access
+// Every grid access goes through a global (address space 1) pointer.  A pointer
+// read from memory (a batch table) is otherwise generic and becomes a flat
+// access, which also counts against the scalar-memory counter: each scalar load
+// would then wait for every load in flight.
+template <typename T> struct RawBits { using type = T; };
+template <> struct RawBits<bf16_t> { using type = uint16_t; };
+template <> struct RawBits<f16_t> { using type = uint16_t; };
+
+template <typename T> __device__ __forceinline__ T gld(const void* p) {
+    using R = typename RawBits<T>::type;
+    const R r = *(const __attribute__((address_space(1))) R*)p;
+    T v;
+    __builtin_memcpy(&v, &r, sizeof(T));
+    return v;
+}
+template <typename T> __device__ __forceinline__ void gst(void* p, T v) {
+    using R = typename RawBits<T>::type;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof(T));
+    *(__attribute__((address_space(1))) R*)p = r;
+}
+
 // ---------------------------------------------------------------- strip I/O
 // One lane's V consecutive elements of a row.  `full` = the whole strip lies
 // inside [0, width) and is aligned for a single vector access.
@@ -96,7 +119,7 @@ template <typename S, int V> struct VecBits {
 template <typename S, int V>
 __device__ __forceinline__ void load_strip_vec(const S* __restrict__ p, typename Storage<S>::acc (&out)[V]) {
     using VB = typename VecBits<S, V>::type;
-    VB raw = *reinterpret_cast<const VB*>(p);
+    VB raw = gld<VB>(p);
     S tmp[V];
     __builtin_memcpy(tmp, &raw, sizeof(tmp));
 #pragma unroll
@@ -111,7 +134,7 @@ __device__ __forceinline__ void store_strip_vec(S* __restrict__ p, const typenam
     for (int j = 0; j < V; ++j) tmp[j] = Storage<S>::store(v[j]);
     VB raw;
     __builtin_memcpy(&raw, tmp, sizeof(tmp));
-    *reinterpret_cast<VB*>(p) = raw;
+    gst<VB>(p, raw);
 }
 
 // Boundary rules for reads outside the grid.

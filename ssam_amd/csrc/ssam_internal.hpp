@@ -7,12 +7,17 @@
 
 namespace ssam {
 
+// 2D kernel shapes.  kBoxKahan: box / dense taps with compensated accumulation
+// (each filter row is summed on its own, then added to the running sum with
+// Kahan compensation).
+enum Shape2D : int { kStar = 0, kBox = 1, kBoxKahan = 2 };
+
 // A validated 2D request: one launch of `steps` fused sweeps (steps in 1,2,4,8).
 struct Req2D {
     const void* src;
     void* dst;
     int64_t width, height, pitch;
-    int shape;       // Shape2D (kStar / kBox; DENSE arrives embedded in a box)
+    int shape;       // Shape2D (DENSE arrives embedded in a box)
     int radius;      // filter half-width after embedding
     int steps;       // fused steps of this launch
     int boundary;    // Boundary

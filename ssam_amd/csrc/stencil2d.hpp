@@ -33,10 +33,9 @@
 //    (vertically adjacent segments share that XCD's L2 for their halo rows).
 #pragma once
 #include "ssam_device.hpp"
+#include "ssam_internal.hpp"
 
 namespace ssam {
-
-enum Shape2D : int { kStar = 0, kBox = 1 };
 
 template <typename A, int NT> struct Taps { A w[NT]; };
 
@@ -87,13 +86,21 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const A*
                                             A (&out)[V]) {
     constexpr int K = 2 * R + 1;
     constexpr int D = (R + V - 1) / V;  // largest lane distance
+    constexpr bool KAHAN = SHAPE == kBoxKahan;
     A P[V][2 * D + 1];
+    A C[V][2 * D + 1];  // Kahan compensation (KAHAN only; dead code otherwise)
 #pragma unroll
     for (int j = 0; j < V; ++j)
 #pragma unroll
-        for (int d = 0; d < 2 * D + 1; ++d) P[j][d] = 0;
+        for (int d = 0; d < 2 * D + 1; ++d) P[j][d] = C[j][d] = 0;
 #pragma unroll
     for (int n = 0; n < K; ++n) {
+        A Q[V][2 * D + 1];  // this filter row's partials (KAHAN, rows after the first)
+        const bool fresh = KAHAN && n > 0;
+#pragma unroll
+        for (int j = 0; j < V; ++j)
+#pragma unroll
+            for (int d = 0; d < 2 * D + 1; ++d) Q[j][d] = 0;
 #pragma unroll
         for (int m = -R; m <= R; ++m) {
             if (SHAPE == kStar && n != R && m != 0) continue;
@@ -102,9 +109,33 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const A*
             for (int j = 0; j < V; ++j) {
                 const int c = j + m;
                 const int d = c >= 0 ? c / V : -((-c + V - 1) / V);
-                P[j][d + D] = fmaa(wt, X[n][c - d * V], P[j][d + D]);
+                if (fresh)
+                    Q[j][d + D] = fmaa(wt, X[n][c - d * V], Q[j][d + D]);
+                else
+                    P[j][d + D] = fmaa(wt, X[n][c - d * V], P[j][d + D]);
             }
         }
+        if (fresh) {
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+#pragma unroll
+                for (int d = 0; d < 2 * D + 1; ++d) {
+                    // partials this row touched: lane distances floor((j-R)/V)..floor((j+R)/V)
+                    const int lo = j - R >= 0 ? (j - R) / V : -((R - j + V - 1) / V);
+                    const int hi = (j + R) / V;
+                    if (d - D < lo || d - D > hi) continue;
+                    const A y = Q[j][d] - C[j][d];
+                    const A t = P[j][d] + y;
+                    C[j][d] = (t - P[j][d]) - y;
+                    P[j][d] = t;
+                }
+        }
+    }
+    if (KAHAN) {
+#pragma unroll
+        for (int j = 0; j < V; ++j)
+#pragma unroll
+            for (int d = 0; d < 2 * D + 1; ++d) P[j][d] -= C[j][d];
     }
 #pragma unroll
     for (int j = 0; j < V; ++j) {
@@ -202,10 +233,10 @@ struct Stencil2D {
                     if (clamp) {
                         const uint32_t o = static_cast<uint32_t>(clampi(x, 0, L.p.width - 1)) *
                                            static_cast<uint32_t>(sizeof(S));
-                        dst[j] = Storage<S>::load(*reinterpret_cast<const S*>(row + o));
+                        dst[j] = Storage<S>::load(gld<S>(row + o));
                     } else {
                         const uint32_t o = L.xoff + static_cast<uint32_t>(j * sizeof(S));
-                        dst[j] = L.inb[j] ? Storage<S>::load(*reinterpret_cast<const S*>(row + o))
+                        dst[j] = L.inb[j] ? Storage<S>::load(gld<S>(row + o))
                                           : A(0);
                     }
                 }
@@ -224,8 +255,8 @@ struct Stencil2D {
 #pragma unroll
             for (int j = 0; j < V; ++j)
                 if (L.inb[j])
-                    *reinterpret_cast<S*>(row + L.xoff + static_cast<uint32_t>(j * sizeof(S))) =
-                        Storage<S>::store(v[j]);
+                    gst<S>(row + L.xoff + static_cast<uint32_t>(j * sizeof(S)),
+                        Storage<S>::store(v[j]));
         }
     }
 

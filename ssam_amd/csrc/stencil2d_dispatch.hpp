@@ -97,7 +97,7 @@ static int dispatch_radius(const Req2D& rq, hipStream_t st) {
         default:
             break;
     }
-    if constexpr (SHAPE == kBox && T == 1) {
+    if constexpr (SHAPE != kStar && T == 1) {
         switch (rq.radius) {
 #define SSAM_R(r) \
     case r:       \
@@ -116,6 +116,8 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
     constexpr int V = Lanes2D<S>::V, VT1 = Lanes2D<S>::VT1, VT2 = Lanes2D<S>::VT2;
     if (rq.steps == 1) {
         if (rq.shape == kStar) return dispatch_radius<S, kStar, V, 1>(rq, st);
+        // FP64 storage accumulates in FP64: compensation changes nothing there
+        if (rq.shape == kBoxKahan && sizeof(S) != 8) return dispatch_radius<S, kBoxKahan, V, 1>(rq, st);
         return dispatch_radius<S, kBox, V, 1>(rq, st);
     }
     // fused time steps: star radius 1 and 2 (the reference's temporal 5-point and

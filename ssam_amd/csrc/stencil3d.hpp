@@ -100,13 +100,13 @@ struct Stencil3D {
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
                     const uint32_t x = static_cast<uint32_t>(clampi(c.x0 + j, 0, c.p.nx - 1));
-                    P.v[i][j] = Storage<S>::load(*reinterpret_cast<const S*>(row + x * sizeof(S)));
+                    P.v[i][j] = Storage<S>::load(gld<S>(row + x * sizeof(S)));
                 }
             }
 #pragma unroll
             for (int k = 0; k < R; ++k) {
                 const uint32_t hx = static_cast<uint32_t>(min(c.hx + k, c.p.nx - 1));
-                P.halo[i][k] = Storage<S>::load(*reinterpret_cast<const S*>(row + hx * sizeof(S)));
+                P.halo[i][k] = Storage<S>::load(gld<S>(row + hx * sizeof(S)));
             }
         }
     }
@@ -168,8 +168,8 @@ struct Stencil3D {
 #pragma unroll
                     for (int j = 0; j < V; ++j)
                         if (c.x0 + j < c.p.nx && c.x0 + j >= 0)
-                            *reinterpret_cast<S*>(row + c.xoff + static_cast<uint32_t>(j * sizeof(S))) =
-                                Storage<S>::store(out[j]);
+                            gst<S>(row + c.xoff + static_cast<uint32_t>(j * sizeof(S)),
+                                Storage<S>::store(out[j]));
                 }
             }
         }

@@ -79,6 +79,18 @@ def test_stencil2d_rejects_malformed_requests():
     assert _s2d(dd) == L.E_UNSUPPORTED
 
 
+def test_stencil2d_flags():
+    dd, _ = _desc(shape=L.BOX)
+    dd.flags = 2                                            # unknown flag bit
+    assert _s2d(dd) == L.E_INVALID
+    dd, _ = _desc(shape=L.STAR)                             # compensated star
+    dd.flags = L.FLAG_COMPENSATED
+    assert _s2d(dd) == L.E_UNSUPPORTED
+    dd, _ = _desc(shape=L.BOX, steps=2)                     # compensated fused steps
+    dd.flags = L.FLAG_COMPENSATED
+    assert _s2d(dd, work=FAKE_W) == L.E_UNSUPPORTED
+
+
 def test_reference_entries_validate():
     lib = L.lib()
     assert lib.ssam_j2dstar_f32(FAKE_A, FAKE_B, 64, 64, 1, None, None) == L.E_INVALID
@@ -169,3 +181,5 @@ def test_batch_plan_validates():
     b.storage, b.steps = L.F32, 4            # 3D multi-step without work
     assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
     assert lib.ssam_batch_run(None, None, 0) == L.E_INVALID
+    assert lib.ssam_batch_plan_tune(None, None) == L.E_INVALID
+    assert lib.ssam_batch_plan_tiling(None, None, None, None) == L.E_INVALID

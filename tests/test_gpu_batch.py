@@ -28,8 +28,7 @@ def _check2d(got, x, f, tol):
     assert e["max_rel"] <= tol, e
 
 
-@pytest.mark.parametrize("storage,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2)])
-def test_batch_mixed_2d_3d(storage, tol):
+def _mixed_items(storage):
     specs2d = [(37, 251, "star", 1), (300, 129, "box", 2), (64, 64, "box", 5), (1, 7, "star", 1),
                (250, 997, "box", 1), (513, 300, "box", 5), (45, 600, "star", 3), (129, 130, "box", 2)]
     items, refs = [], []
@@ -51,6 +50,28 @@ def test_batch_mixed_2d_3d(storage, tol):
         items.append(dict(ndim=3, src=src, dst=dst, work=work, coeffs=coef, steps=3,
                           shape=L.FACE_EDGE if shape == "face_edge" else L.STAR, radius=r))
         refs.append(("3d", dst, x, (taps, coef, r)))
+    return items, refs
+
+
+def _check_refs(refs, tol):
+    for kind, dst, x, f in refs:
+        got = dst.to(torch.float64).cpu().numpy()
+        if kind == "2d":
+            _check2d(got, x, f, tol)
+        else:
+            taps, coef, r = f
+            cur = x.astype(np.float64)
+            s = np.abs(cur)
+            for _ in range(3):
+                cur = oracle.stencil3d_taps(cur, taps, coef, r)
+                s = oracle.stencil3d_taps(s, taps, np.abs(coef), r)
+            e = oracle.err(got, cur, s)
+            assert e["max_rel"] <= 3e-5, e
+
+
+@pytest.mark.parametrize("storage,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2)])
+def test_batch_mixed_2d_3d(storage, tol):
+    items, refs = _mixed_items(storage)
     plan = api.BatchPlan(items)
     t2, t3, pts = plan.info()
     assert t2 > 0 and t3 > 0 and pts > 0
@@ -59,19 +80,40 @@ def test_batch_mixed_2d_3d(storage, tol):
             dst.fill_(float("nan"))
         plan.run(eager=(rep == 1))
         torch.cuda.synchronize()
-        for kind, dst, x, f in refs:
-            got = dst.to(torch.float64).cpu().numpy()
-            if kind == "2d":
-                _check2d(got, x, f, tol)
-            else:
-                taps, coef, r = f
-                cur = x.astype(np.float64)
-                s = np.abs(cur)
-                for _ in range(3):
-                    cur = oracle.stencil3d_taps(cur, taps, coef, r)
-                    s = oracle.stencil3d_taps(s, taps, np.abs(coef), r)
-                e = oracle.err(got, cur, s)
-                assert e["max_rel"] <= 3e-5, e
+        _check_refs(refs, tol)
+    plan.close()
+
+
+@pytest.mark.parametrize("seg", ["8", "16", "300"])
+def test_batch_2d_tile_heights(seg, monkeypatch):
+    """Short and long 2D tiles (many runs per item across the XCD queues, one tile
+    per strip); every output written once, against the oracle."""
+    monkeypatch.setenv("SSAM_BATCH2D_SEG", seg)
+    items, refs = _mixed_items(torch.float32)
+    plan = api.BatchPlan(items)
+    assert plan.tiling()["seg_rows_2d"]["box5"] == int(seg)
+    for _, dst, _, _ in refs:
+        dst.fill_(float("nan"))
+    plan.run()
+    torch.cuda.synchronize()
+    _check_refs(refs, 1e-5)
+    plan.close()
+
+
+def test_batch_tune_keeps_results():
+    """ssam_batch_plan_tune picks a tiling from its candidates; runs after it (graph
+    recaptured, eager) still match the oracle."""
+    items, refs = _mixed_items(torch.float32)
+    plan = api.BatchPlan(items)
+    t = plan.tune()
+    assert set(t["seg_rows_2d"].values()) <= {32, 64, 128, 256}
+    assert t["variant_3d"] in (0, 2, 3, 4) and t["zseg_3d"] in (16, 32)
+    for rep in range(2):
+        for _, dst, _, _ in refs:
+            dst.fill_(float("nan"))
+        plan.run(eager=(rep == 1))
+        torch.cuda.synchronize()
+        _check_refs(refs, 1e-5)
     plan.close()
 
 
@@ -83,3 +125,45 @@ def test_config4_generator_shapes():
         assert max(g["w"], g["h"]) <= 4 * min(g["w"], g["h"])
     assert {g["r"] for g in g2} == {1, 2, 5}
     assert all(64 <= g[k] <= 256 for g in g3 for k in ("nx", "ny", "nz"))
+
+
+_SIZES3 = [(70, 33, 20), (1, 1, 1), (2, 3, 2), (3, 1, 5), (63, 5, 4), (62, 2, 3), (125, 9, 7),
+           (124, 8, 9), (200, 17, 33), (64, 64, 12)]
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("storage", [torch.float32, torch.float64])
+def test_batch3d_tile_shapes(variant, storage, monkeypatch):
+    """Every 3D tile shape of the batched launch (columns / rows per lane, prefetch
+    depth) on tiny, odd-width and multi-strip grids, short z segments, two step
+    counts (two groups), graph replay; against the oracle."""
+    monkeypatch.setenv("SSAM_BATCH3D_VARIANT", str(variant))
+    monkeypatch.setenv("SSAM_BATCH3D_ZSEG", "5")
+    items, refs = [], []
+    for k, (nx, ny, nz) in enumerate(_SIZES3):
+        shape = "face_edge" if k % 3 == 1 else "star"
+        steps = 3 if k % 2 == 0 else 2
+        x = oracle.uniform(500 + k, nx * ny * nz, 0, 1, np.float64).reshape(nz, ny, nx)
+        src = torch.from_numpy(x).to(storage).cuda()
+        dst = torch.full_like(src, float("nan"))
+        work = torch.empty_like(src)
+        taps = wl.taps3d(shape, 1)
+        coef = wl.uniform(600 + k, len(taps), -0.5, 0.5)
+        items.append(dict(ndim=3, src=src, dst=dst, work=work, coeffs=coef, steps=steps,
+                          shape=L.FACE_EDGE if shape == "face_edge" else L.STAR, radius=1))
+        refs.append((dst, src.to(torch.float64).cpu().numpy(), taps, coef, steps))
+    plan = api.BatchPlan(items)
+    tol = 3e-5 if storage == torch.float32 else 1e-12
+    for rep in range(2):
+        for dst, *_ in refs:
+            dst.fill_(float("nan"))
+        plan.run(eager=(rep == 0))
+        torch.cuda.synchronize()
+        for dst, x, taps, coef, steps in refs:
+            cur, s = x, np.abs(x)
+            for _ in range(steps):
+                cur = oracle.stencil3d_taps(cur, taps, coef, 1)
+                s = oracle.stencil3d_taps(s, taps, np.abs(coef), 1)
+            e = oracle.err(dst.to(torch.float64).cpu().numpy(), cur, s)
+            assert e["max_rel"] <= tol, (variant, x.shape, e)
+    plan.close()

@@ -189,3 +189,13 @@ def test_err_metrics():
     e = oracle.err(a, b, np.array([1.0, 1.0, 2.0]))
     assert e["max_abs"] == 1.0 and e["match"] == 1 and e["count"] == 3
     assert e["max_rel"] == 0.5 and np.isclose(e["rms"], np.sqrt((0.25 + 1) / 3))
+
+
+def test_err_flags_nan():
+    """An unwritten output (NaN) fails every tolerance, whatever the scale."""
+    a = np.array([1.0, np.nan, 3.0])
+    b = np.array([1.0, 2.0, 3.0])
+    e = oracle.err(a, b, np.array([1.0, 1.0, 1.0]))
+    assert e["max_rel"] == np.inf and e["max_abs"] == np.inf and e["match"] == 2
+    e = oracle.err(b, a)
+    assert e["max_rel"] == np.inf
