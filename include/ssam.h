@@ -229,13 +229,16 @@ int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles
 /* Autotune the plan's tiling on its own buffers (dst and work are overwritten;
  * the stream is synchronised): 2D tile height per shape (32..256 rows) and the 3D
  * tile shape (columns and rows per lane, prefetch depth) and z-segment length,
- * each timed over eager runs; the fastest is kept and the next ssam_batch_run
- * captures a new graph. */
+ * and the queue affinity of each, timed over eager runs; the fastest is kept
+ * and the next ssam_batch_run captures a new graph. */
 int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
 /* The plan's tiling: 2D tile height per shape (4 entries: 5-point star, 3x3,
- * 5x5, 11x11 box), 3D tile-shape index and z-segment length (any may be NULL). */
+ * 5x5, 11x11 box), 3D tile-shape index and z-segment length, and the queue
+ * affinity of the 2D and 3D launches (2 entries; 1: waves start at their own
+ * XCD's queue, 0: all waves work through the tiles in table order).  Any
+ * pointer may be NULL. */
 int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* variant_3d,
-                           int32_t* zseg_3d);
+                           int32_t* zseg_3d, int32_t* affinity);
 void ssam_batch_plan_destroy(ssam_batch_plan plan);
 
 /* ---------------------------------------------------------------- autotuner

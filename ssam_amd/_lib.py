@@ -114,7 +114,8 @@ _PROTOS = {
     "ssam_batch_plan_destroy": (None, [ctypes.c_void_p]),
     "ssam_batch_plan_tune": (_i, [ctypes.c_void_p, _vp]),
     "ssam_batch_plan_tiling": (_i, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
-                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_int32)]),
     "ssam_fill_uniform": (_i, [_vp, _i, _i64, _u64, _u64, _d, _d, _vp]),
     "ssam_stream_copy": (_i, [_vp, _vp, _i64, _vp]),
 }

@@ -15,12 +15,13 @@
 //    atomic from lane 0), so neighbouring tiles run under one L2 and share their
 //    halo rows there; when its queue is empty it takes tiles from the other queues
 //    in turn (the result never depends on which XCD runs what), and leaves when all
-//    are empty.
+//    are empty.  With affinity off every wave starts at queue 0 and the device
+//    works through the table in order (the tuner picks per launch kind).
 //  * Graph: counter reset (a fill kernel, no memset), the 2D launches, the per-step
 //    3D launches and any per-item fallback launches are captured once on a private
 //    stream and replayed on the caller's stream.
-//  * Tuner: ssam_batch_plan_tune times candidate tile heights per 2D shape and 3D
-//    tile shapes on the plan's own buffers and keeps the fastest.
+//  * Tuner: ssam_batch_plan_tune times candidate tile heights per 2D shape, 3D
+//    tile shapes and queue affinity on the plan's own buffers, keeps the fastest.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -80,7 +81,12 @@ constexpr int kBatchWavesPerBlock = 4;
 constexpr int kQueues = 8;  // one per XCD
 
 // First tile index of each queue (queue q holds tiles [beg[q], beg[q+1])).
-struct QueueSet { int beg[kQueues + 1]; };
+// affinity 1: a wave starts at its own XCD's queue; 0: every wave starts at
+// queue 0, so the whole device works through the tiles in table order.
+struct QueueSet {
+    int beg[kQueues + 1];
+    int affinity;
+};
 
 // Physical XCD of this wave (hardware register XCC_ID, bits 3:0).
 __device__ __forceinline__ int xcd_id() {
@@ -97,7 +103,7 @@ __device__ __forceinline__ int next_tile(int* counter) {
 // until every queue is empty.  counters: one per queue, zero at launch.
 template <typename F>
 __device__ __forceinline__ void drain_queues(const QueueSet& qs, int* __restrict__ counters, F&& body) {
-    const int x = xcd_id();
+    const int x = qs.affinity ? xcd_id() : 0;
     for (int k = 0; k < kQueues; ++k) {
         const int q = (x + k) & (kQueues - 1);
         const int beg = qs.beg[q], end = qs.beg[q + 1];
@@ -139,8 +145,8 @@ struct Var3D { int v, ny, pf; };
 constexpr Var3D kVar3D[] = {{1, 8, 2}, {1, 4, 3}, {2, 4, 2}, {2, 8, 1},
                             {2, 8, 2}, {4, 4, 1}, {4, 4, 2}, {4, 8, 1}};
 constexpr int kNumVar3D = 8;
-constexpr int kDefaultVar3D = 3;
-constexpr int kDefaultZseg3D = 16;
+constexpr int kDefaultVar3D = 4;
+constexpr int kDefaultZseg3D = 32;
 constexpr int kDefaultSeg2D = 64;
 
 template <typename S, int VAR> struct Var3DT {
@@ -203,6 +209,7 @@ struct ssam_batch_plan_s {
     std::vector<int> fallback;                // indices into items
     int seg2d[kNumKind2D] = {kDefaultSeg2D, kDefaultSeg2D, kDefaultSeg2D, kDefaultSeg2D};
     int var3d = kDefaultVar3D, zseg3d = kDefaultZseg3D;
+    int aff2d = 1, aff3d = 0;  // queue affinity (QueueSet::affinity)
     int ncounters = 0;
     void* d_counters = nullptr;
     int64_t tiles2d = 0, tiles3d = 0, point_updates = 0;
@@ -400,6 +407,7 @@ int build_group2d(Plan* P, Plan::Group& g) {
     }
     std::vector<int4> tiles;
     pack_queues(per, &tiles, &g.qs);
+    g.qs.affinity = P->aff2d;
     free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
     g.blocks = resident_blocks(batch2d_fn<S>());
@@ -454,6 +462,7 @@ int build_group3d(Plan* P, Plan::Group& g) {
     }
     std::vector<int4> tiles;
     pack_queues(per, &tiles, &g.qs);
+    g.qs.affinity = P->aff3d;
     free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
     g.blocks = resident_blocks(batch3d_fn<S>(variant, g.kind));
@@ -690,6 +699,8 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
     P->zseg3d = std::max(1, std::min(env_int("SSAM_BATCH3D_ZSEG", kDefaultZseg3D), 2047));
     const int seg = env_int("SSAM_BATCH2D_SEG", kDefaultSeg2D);
     for (int k = 0; k < kNumKind2D; ++k) P->seg2d[k] = seg >= 8 ? seg : kDefaultSeg2D;
+    P->aff2d = env_int("SSAM_BATCH2D_AFFINITY", 1) != 0;
+    P->aff3d = env_int("SSAM_BATCH3D_AFFINITY", 0) != 0;
     for (int s = 0; s < 4; ++s) {
         if (g2[s].empty()) continue;
         Plan::Group g;
@@ -777,7 +788,8 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
     }
     float ms = 0.f;
     int rc = SSAM_OK;
-    // 2D: tile height per shape, one shape at a time (the others held)
+    // 2D: tile height per shape, one shape at a time (the others held), then
+    // queue affinity
     const int segs[] = {32, 64, 128, 256};
     for (int k = 0; k < kNumKind2D && has2 && rc == SSAM_OK; ++k) {
         if (!kinds[k]) continue;
@@ -795,37 +807,56 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
         }
         P->seg2d[k] = best;
     }
-    // 3D: tile shape x z-segment length
-    if (has3 && rc == SSAM_OK) {
-        const int vars[] = {0, 2, 3, 4};
-        const int zsegs[] = {16, 32};
-        int bv = P->var3d, bz = P->zseg3d;
+    if (has2 && rc == SSAM_OK) {
+        int best = P->aff2d;
         float best_ms = 1e30f;
-        for (int v : vars) {
-            for (int z : zsegs) {
-                P->var3d = v;
-                P->zseg3d = z;
-                rc = build_all(P);
-                if (rc == SSAM_OK) rc = time_parts(P, st, kRun3D, reps, &ms);
-                if (rc != SSAM_OK) break;
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    bv = v;
-                    bz = z;
-                }
+        for (int a = 0; a < 2 && rc == SSAM_OK; ++a) {
+            P->aff2d = a;
+            rc = build_all(P);
+            if (rc == SSAM_OK) rc = time_parts(P, st, kRun2D, reps, &ms);
+            if (rc == SSAM_OK && ms < best_ms) {
+                best_ms = ms;
+                best = a;
             }
-            if (rc != SSAM_OK) break;
         }
+        P->aff2d = best;
+    }
+    // 3D: tile shape x z-segment length x queue affinity
+    if (has3 && rc == SSAM_OK) {
+        const int vars[] = {2, 3, 4};
+        const int zsegs[] = {16, 32};
+        int bv = P->var3d, bz = P->zseg3d, ba = P->aff3d;
+        float best_ms = 1e30f;
+        for (int v : vars)
+            for (int z : zsegs)
+                for (int a = 0; a < 2 && rc == SSAM_OK; ++a) {
+                    P->var3d = v;
+                    P->zseg3d = z;
+                    P->aff3d = a;
+                    rc = build_all(P);
+                    if (rc == SSAM_OK) rc = time_parts(P, st, kRun3D, reps, &ms);
+                    if (rc == SSAM_OK && ms < best_ms) {
+                        best_ms = ms;
+                        bv = v;
+                        bz = z;
+                        ba = a;
+                    }
+                }
         P->var3d = bv;
         P->zseg3d = bz;
+        P->aff3d = ba;
     }
     const int rc2 = build_all(P);
     return rc != SSAM_OK ? rc : rc2;
 }
 
 int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* variant_3d,
-                           int32_t* zseg_3d) {
+                           int32_t* zseg_3d, int32_t* affinity) {
     if (!plan) return SSAM_E_INVALID;
+    if (affinity) {
+        affinity[0] = plan->aff2d;
+        affinity[1] = plan->aff3d;
+    }
     if (seg_rows_2d)
         for (int k = 0; k < kNumKind2D; ++k) seg_rows_2d[k] = plan->seg2d[k];
     if (variant_3d) *variant_3d = plan->var3d;

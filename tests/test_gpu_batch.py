@@ -107,7 +107,8 @@ def test_batch_tune_keeps_results():
     plan = api.BatchPlan(items)
     t = plan.tune()
     assert set(t["seg_rows_2d"].values()) <= {32, 64, 128, 256}
-    assert t["variant_3d"] in (0, 2, 3, 4) and t["zseg_3d"] in (16, 32)
+    assert t["variant_3d"] in (2, 3, 4) and t["zseg_3d"] in (16, 32)
+    assert t["affinity_2d"] in (0, 1) and t["affinity_3d"] in (0, 1)
     for rep in range(2):
         for _, dst, _, _ in refs:
             dst.fill_(float("nan"))
@@ -131,13 +132,14 @@ _SIZES3 = [(70, 33, 20), (1, 1, 1), (2, 3, 2), (3, 1, 5), (63, 5, 4), (62, 2, 3)
            (124, 8, 9), (200, 17, 33), (64, 64, 12)]
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant,affinity", [(v, v % 2) for v in range(8)])
 @pytest.mark.parametrize("storage", [torch.float32, torch.float64])
-def test_batch3d_tile_shapes(variant, storage, monkeypatch):
+def test_batch3d_tile_shapes(variant, affinity, storage, monkeypatch):
     """Every 3D tile shape of the batched launch (columns / rows per lane, prefetch
     depth) on tiny, odd-width and multi-strip grids, short z segments, two step
     counts (two groups), graph replay; against the oracle."""
     monkeypatch.setenv("SSAM_BATCH3D_VARIANT", str(variant))
+    monkeypatch.setenv("SSAM_BATCH3D_AFFINITY", str(affinity))
     monkeypatch.setenv("SSAM_BATCH3D_ZSEG", "5")
     items, refs = [], []
     for k, (nx, ny, nz) in enumerate(_SIZES3):
