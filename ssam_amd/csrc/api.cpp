@@ -82,6 +82,7 @@ struct TuneVal {
     int waves_per_simd;  // tile-count target
     int tchunk;          // fused steps per launch (<= kernel maximum)
     float ms;            // measured time of the chosen form
+    int form;            // Form2D (kFormAuto: the dispatch heuristic)
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune;
@@ -101,6 +102,7 @@ bool tune_lookup(const TuneKey& k, TuneVal* v) {
 // Forced tiling for one call (0 = cache / heuristic); used by the tuner.
 thread_local int t_force_wps = 0;
 thread_local int t_force_tchunk = 0;
+thread_local int t_force_form = 0;
 
 int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int64_t width,
                    int64_t height, int64_t pitch, void* work, hipStream_t st) {
@@ -160,7 +162,7 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
 
     // split the steps into fused launches (8, 4, 2, 1) chained through work/dst
     const TuneKey key{d->storage, kshape, R, d->steps, width, height};
-    TuneVal tv{kDefaultWavesPerSimd, 0, 0.f};
+    TuneVal tv{kDefaultWavesPerSimd, 0, 0.f, kFormAuto};
     const bool tuned = tune_lookup(key, &tv);
     int fmax = max_fused(d->storage, shape, R);
     if (t_force_tchunk > 0) fmax = t_force_tchunk < fmax ? t_force_tchunk : fmax;
@@ -190,6 +192,7 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
     rq.waves_per_simd = t_force_wps > 0 ? t_force_wps
                         : env_int("SSAM_WAVES_PER_SIMD", tuned ? tv.waves_per_simd
                                                                : kDefaultWavesPerSimd);
+    rq.form = t_force_form > 0 ? t_force_form : env_int("SSAM_2D_FORM", tuned ? tv.form : kFormAuto);
     const size_t n = chunks.size();
     const void* in = src;
     for (size_t i = 0; i < n; ++i) {
@@ -582,11 +585,18 @@ int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst
     }
     HipEvents ev;
     if (!ev.a || !ev.b) return SSAM_E_HIP;
-    TuneVal best{kDefaultWavesPerSimd, 0, 1e30f};
+    TuneVal best{kDefaultWavesPerSimd, 0, 1e30f, kFormAuto};
+    // matrix-core candidates where the dispatch may use them (BF16/FP16 boxes)
+    std::vector<int> form_c{kFormValu};
+    if ((d->storage == SSAM_BF16 || d->storage == SSAM_F16) && d->shape != SSAM_SHAPE_STAR &&
+        !(d->flags & SSAM_FLAG_COMPENSATED) && d->steps == 1)
+        form_c.push_back(kFormMfma);
+    for (int form : form_c)
     for (int tc : tch_c) {
         for (int wps : wps_c) {
             t_force_wps = wps;
             t_force_tchunk = tc;
+            t_force_form = form;
             float ms_sum = 0.f;
             bool ok = true;
             rc = stencil2d_impl(d, src, dst, width, height, pitch, work, st);  // warm
@@ -603,9 +613,10 @@ int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst
             }
             t_force_wps = 0;
             t_force_tchunk = 0;
+            t_force_form = 0;
             if (!ok) return SSAM_E_HIP;
             if (rc != SSAM_OK) return rc;
-            if (ms_sum / 3 < best.ms) best = TuneVal{wps, tc, ms_sum / 3};
+            if (ms_sum / 3 < best.ms) best = TuneVal{wps, tc, ms_sum / 3, form};
         }
     }
     {
@@ -627,12 +638,12 @@ int ssam_tune_save(const char* path) {
     FILE* fp = std::fopen(path, "w");
     if (!fp) return SSAM_E_INVALID;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    std::fprintf(fp, "# ssam autotune v1: storage shape radius steps width height waves_per_simd tchunk ms\n");
+    std::fprintf(fp, "# ssam autotune v2: storage shape radius steps width height waves_per_simd tchunk ms form\n");
     for (const auto& kv : g_tune)
-        std::fprintf(fp, "%d %d %d %d %lld %lld %d %d %.6f\n", kv.first.storage, kv.first.shape,
+        std::fprintf(fp, "%d %d %d %d %lld %lld %d %d %.6f %d\n", kv.first.storage, kv.first.shape,
                      kv.first.radius, kv.first.steps, static_cast<long long>(kv.first.w),
                      static_cast<long long>(kv.first.h), kv.second.waves_per_simd, kv.second.tchunk,
-                     static_cast<double>(kv.second.ms));
+                     static_cast<double>(kv.second.ms), kv.second.form);
     std::fclose(fp);
     return SSAM_OK;
 }
@@ -646,13 +657,13 @@ int ssam_tune_load(const char* path) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     while (std::fgets(line, sizeof(line), fp)) {
         if (line[0] == '#') continue;
-        int sto, sh, r, steps, wps, tc;
+        int sto, sh, r, steps, wps, tc, form = kFormAuto;
         long long w, h;
         double ms;
-        if (std::sscanf(line, "%d %d %d %d %lld %lld %d %d %lf", &sto, &sh, &r, &steps, &w, &h, &wps,
-                        &tc, &ms) == 9 &&
-            wps > 0 && wps <= 64 && tc >= 0) {
-            g_tune[TuneKey{sto, sh, r, steps, w, h}] = TuneVal{wps, tc, static_cast<float>(ms)};
+        const int nf = std::sscanf(line, "%d %d %d %d %lld %lld %d %d %lf %d", &sto, &sh, &r, &steps, &w,
+                                   &h, &wps, &tc, &ms, &form);
+        if (nf >= 9 && wps > 0 && wps <= 64 && tc >= 0 && form >= kFormAuto && form <= kFormMfma) {
+            g_tune[TuneKey{sto, sh, r, steps, w, h}] = TuneVal{wps, tc, static_cast<float>(ms), form};
             ++n;
         }
     }

@@ -24,7 +24,11 @@ struct Req2D {
     const double* taps;  // (2r+1)^2 row-major, host
     int seg_rows;        // 0 = heuristic
     int waves_per_simd;  // tile-count target for the heuristic
+    int form;            // Form2D: kernel family (auto picks the matrix cores where they apply)
 };
+
+// 2D kernel family: register-cache VALU kernels or the matrix-core (MFMA) form.
+enum Form2D : int { kFormAuto = 0, kFormValu = 1, kFormMfma = 2 };
 
 template <typename S> int dispatch2d(const Req2D& rq, hipStream_t st);
 

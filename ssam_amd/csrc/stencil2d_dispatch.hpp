@@ -3,6 +3,7 @@
 // tiles and converts the taps to the accumulation type.
 #pragma once
 #include "stencil2d.hpp"
+#include "stencil2d_mfma.hpp"
 #include "ssam_internal.hpp"
 
 namespace ssam {
@@ -115,6 +116,7 @@ template <typename S>
 int dispatch2d(const Req2D& rq, hipStream_t st) {
     constexpr int V = Lanes2D<S>::V, VT1 = Lanes2D<S>::VT1, VT2 = Lanes2D<S>::VT2;
     if (rq.steps == 1) {
+        if (rq.form != kFormValu && mfma_eligible<S>(rq)) return launch_mfma<S>(rq, st);
         if (rq.shape == kStar) return dispatch_radius<S, kStar, V, 1>(rq, st);
         // FP64 storage accumulates in FP64: compensation changes nothing there
         if (rq.shape == kBoxKahan && sizeof(S) != 8) return dispatch_radius<S, kBoxKahan, V, 1>(rq, st);

@@ -1,0 +1,343 @@
+// stencil2d_mfma.hpp -- large 2D boxes and filters on the matrix cores (gfx950
+// v_mfma_f32_16x16x32_{bf16,f16}) for BF16 / FP16 storage, FP32 accumulation.
+//
+// Banded-GEMM form of the correlation out(y,x) = sum_{m,n} w[m][n] in(y+m-r, x+n-r):
+// for a 16x16 output tile at (y0, x0) and filter row m,
+//     D(i,j) += sum_k A_m(i,k) B_m(k,j),   A_m(i,k) = in(y0+i+m-r, x0-8+k),
+//     B_m(k,j) = w[m][k-8+r-j]  (0 <= k-8+r-j <= 2r, else 0),  k = 0..31,
+// so one MFMA per filter row covers the 16 + 2r <= 32 input columns of the tile
+// (r <= 8).  The input window starts 8 columns left of the tile, so every lane's
+// A operand (8 consecutive columns of one row) is one aligned 16-byte load.
+//
+// Register cache instead of LDS: lane (i, g) of a 16-lane row group holds, for
+// each of the wave's NT tiles side by side, input rows as 16-row blocks in a
+// ring; A_m for row shift m is built from blocks b and b+1 with two DPP moves
+// (row_shl:m from block b, row_shr:16-m from block b+1) -- the systolic hop of
+// the VALU kernels, run down the rows.  The wave walks down a segment of rows in
+// steps of 16; each step loads one new 16-row block per tile, NB-2 steps ahead.
+//
+// Taps: FP32 from the kernel-argument segment, split into hi + lo storage-type
+// parts (two MFMAs per filter row) so the products carry ~16 mantissa bits of
+// each tap; inputs are exact in the storage type, accumulation is FP32.  Taps
+// that are exact in the storage type use one MFMA per row (SPLIT = false).
+//
+// Output D(i, j) sits in lane j + 16*(i/4), element i%4 (16x16 result layout);
+// rows at or past the segment end and columns past the width are not stored.
+// Zero boundary: rows and columns outside the grid read as 0.
+#pragma once
+#include "ssam_device.hpp"
+#include "ssam_internal.hpp"
+#include "stencil2d.hpp"
+
+namespace ssam {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename S> struct MfmaOp;
+template <> struct MfmaOp<bf16_t> {
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    __device__ static __forceinline__ f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8, a), __builtin_bit_cast(v8, b),
+                                                       c, 0, 0, 0);
+    }
+};
+template <> struct MfmaOp<f16_t> {
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    __device__ static __forceinline__ f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8, a), __builtin_bit_cast(v8, b),
+                                                      c, 0, 0, 0);
+    }
+};
+
+struct ParamsM {
+    int width, height;
+    long long pitch;  // elements between rows
+    int nstrips;      // strips of 16*NT output columns
+    int seg_rows;     // output rows per tile (multiple of 16)
+    int ntiles, nblocks;
+};
+
+constexpr int kWavesPerBlockM = 4;
+
+// Rows shifted up by M inside each 16-lane row group: lane i gets x from lane
+// i+M while i+M < 16, else y from lane i+M-16 (x = rows b, y = rows b+1).
+template <int M> __device__ __forceinline__ uint32_t shift_rows(uint32_t x, uint32_t y) {
+    if constexpr (M == 0) {
+        return x;
+    } else if constexpr (M == 16) {
+        return y;
+    } else {
+        // both moves fill lanes whose source leaves the row group with 0, so the
+        // two halves combine with an OR (foldable into v_or_b32 with a DPP source)
+        const uint32_t lo = static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x100 + M, 0xf, 0xf, true));
+        const uint32_t hi = static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(y), 0x110 + (16 - M), 0xf, 0xf, true));
+        return lo | hi;
+    }
+}
+
+template <typename S, int R, bool SPLIT, int NT>
+struct Mfma2D {
+    static_assert(R >= 1 && R <= 8, "one 32-column window per 16-column tile");
+    static constexpr int K = 2 * R + 1;
+    static constexpr int NB = 4;        // row blocks in the ring (two used + two ahead)
+    static constexpr int OW = 16 * NT;  // output columns per strip
+    using Op = MfmaOp<S>;
+
+    struct Lane {
+        const char* src;
+        char* dst;
+        ParamsM p;
+        int i, g;      // A layout: row i of the block, column chunk g
+        int x0;        // first output column of the strip
+        int ys, ye;    // output rows [ys, ye) of the tile
+        int blast;     // last row block any output row needs
+    };
+
+    // Row block b (rows ys - R + 16 b + i) of tile t: 8 columns from x0+16t-8+8g.
+    template <bool CHK>
+    __device__ static __forceinline__ u32x4 load_block(const Lane& L, int b, int t) {
+        const int r = L.ys - R + 16 * min(b, L.blast) + L.i;
+        const int c = L.x0 + 16 * t - 8 + 8 * L.g;
+        const char* row = L.src + static_cast<long long>(r) * L.p.pitch * static_cast<long long>(sizeof(S));
+        if constexpr (!CHK) {
+            return gld<u32x4>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
+        } else {
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (r >= 0 && r < L.p.height) {
+                if (c >= 0 && c + 8 <= L.p.width) {
+                    v = gld<u32x4>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
+                } else {
+                    uint32_t h[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int x = c + e;
+                        h[e] = (x >= 0 && x < L.p.width)
+                                   ? static_cast<uint32_t>(gld<uint16_t>(row + static_cast<long long>(x) * 2))
+                                   : 0u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v[q] = h[2 * q] | (h[2 * q + 1] << 16);
+                }
+            }
+            return v;
+        }
+    }
+
+    // B operand of filter row m (hi or lo part), built once per wave.
+    __device__ static __forceinline__ void build_taps(const float* __restrict__ w, int j, int g,
+                                                      u32x4 (&bh)[K], u32x4 (&bl)[K]) {
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            uint32_t h[8], l[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int n = 8 * g + e - 8 + R - j;
+                const float v = (n >= 0 && n <= 2 * R) ? w[m * K + n] : 0.f;
+                const S hi = Storage<S>::store(v);
+                const S lo = Storage<S>::store(v - Storage<S>::load(hi));
+                h[e] = hi.bits;
+                l[e] = lo.bits;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                bh[m][q] = h[2 * q] | (h[2 * q + 1] << 16);
+                bl[m][q] = l[2 * q] | (l[2 * q + 1] << 16);
+            }
+        }
+    }
+
+    template <int M>
+    __device__ static __forceinline__ u32x4 shifted(const u32x4& x, const u32x4& y) {
+        u32x4 a;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = shift_rows<M>(x[q], y[q]);
+        return a;
+    }
+
+    // hi and lo parts accumulate separately (independent MFMA chains)
+    template <int M>
+    __device__ static __forceinline__ void rows_from(f32x4 (&D)[NT], f32x4 (&E)[NT], const u32x4 (&X)[NT],
+                                                     const u32x4 (&Y)[NT], const u32x4 (&bh)[K],
+                                                     const u32x4 (&bl)[K]) {
+        if constexpr (M < K) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const u32x4 a = shifted<M>(X[t], Y[t]);
+                D[t] = Op::mma(a, bh[M], D[t]);
+                if constexpr (SPLIT) E[t] = Op::mma(a, bl[M], E[t]);
+            }
+            rows_from<M + 1>(D, E, X, Y, bh, bl);
+        }
+    }
+
+    template <bool CHK>
+    __device__ static __forceinline__ void store_tile(const Lane& L, int y0, int t, const f32x4& d) {
+        const int j = L.i;            // result layout: column = lane % 16
+        const int rg = L.g;           // rows 4*rg .. 4*rg+3
+        const int x = L.x0 + 16 * t + j;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int y = y0 + 4 * rg + q;
+            if (CHK && (y >= L.ye || x >= L.p.width)) continue;
+            if (!CHK && y >= L.ye) continue;
+            char* p = L.dst + (static_cast<long long>(y) * L.p.pitch + x) * static_cast<long long>(sizeof(S));
+            gst<S>(p, Storage<S>::store(d[q]));
+        }
+    }
+
+    template <bool CHK, int KT>
+    __device__ static __forceinline__ void step(const Lane& L, int s0, int nsteps, u32x4 (&ring)[NB][NT],
+                                                const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
+        if constexpr (KT < NB) {
+            const int s = s0 + KT;
+            if (s < nsteps) {
+                // block s + NB - 1 into the slot of block s - 1
+#pragma unroll
+                for (int t = 0; t < NT; ++t) ring[(KT + NB - 1) % NB][t] = load_block<CHK>(L, s + NB - 1, t);
+                f32x4 D[NT], E[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) D[t] = E[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+                rows_from<0>(D, E, ring[KT % NB], ring[(KT + 1) % NB], bh, bl);
+                const int y0 = L.ys + 16 * s;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) store_tile<CHK>(L, y0, t, SPLIT ? D[t] + E[t] : D[t]);
+            }
+            step<CHK, KT + 1>(L, s0, nsteps, ring, bh, bl);
+        }
+    }
+
+    template <bool CHK>
+    __device__ static __forceinline__ void stream(const Lane& L, const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
+        u32x4 ring[NB][NT];
+#pragma unroll
+        for (int b = 0; b < NB - 1; ++b)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) ring[b][t] = load_block<CHK>(L, b, t);
+        const int nsteps = (L.ye - L.ys + 15) / 16;
+        for (int s0 = 0; s0 < nsteps; s0 += NB) step<CHK, 0>(L, s0, nsteps, ring, bh, bl);
+    }
+
+    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const ParamsM& p,
+                               const float* __restrict__ w) {
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+        const int tile = xcd_remap(blockIdx.x, p.nblocks) * kWavesPerBlockM + wv;
+        if (tile >= p.ntiles) return;  // whole wave leaves; no barriers are used
+        const int lane = threadIdx.x & (kWave - 1);
+        Lane L;
+        L.src = reinterpret_cast<const char*>(src);
+        L.dst = reinterpret_cast<char*>(dst);
+        L.p = p;
+        L.i = lane & 15;
+        L.g = lane >> 4;
+        L.x0 = (tile % p.nstrips) * OW;
+        L.ys = (tile / p.nstrips) * p.seg_rows;
+        L.ye = min(p.height, L.ys + p.seg_rows);
+        // output rows up to ye-1 need input rows up to ye-1+R: block index of that row
+        L.blast = (L.ye - 1 + R - (L.ys - R)) / 16;
+        u32x4 bh[K], bl[K];
+        build_taps(w, L.i, L.g, bh, bl);
+        const bool interior = L.x0 - 8 >= 0 && L.x0 + OW + 8 <= p.width && L.ys - R >= 0 &&
+                              L.ys - R + 16 * L.blast + 15 < p.height;
+        if (interior)
+            stream<false>(L, bh, bl);
+        else
+            stream<true>(L, bh, bl);
+    }
+};
+
+template <typename S, int R, bool SPLIT, int NT>
+__global__ __launch_bounds__(kWave * kWavesPerBlockM) __attribute__((amdgpu_waves_per_eu(2)))
+void mfma2d_kernel(Taps<float, (2 * R + 1) * (2 * R + 1)> tp, const S* __restrict__ src,
+                   S* __restrict__ dst, ParamsM p) {
+    (void)tp;
+    const float* __restrict__ w = (const float*)(__builtin_amdgcn_kernarg_segment_ptr());
+    Mfma2D<S, R, SPLIT, NT>::run(src, dst, p, w);
+}
+
+// Host launch for one request; returns SSAM_E_UNSUPPORTED when the request is
+// outside what the matrix-core form covers.
+template <typename S, int R, bool SPLIT>
+int launch_mfma_instance(const Req2D& rq, hipStream_t st) {
+    constexpr int NT = SPLIT ? 2 : 4;  // tiles side by side per wave (register budget)
+    constexpr int K = 2 * R + 1;
+    Taps<float, K * K> tp;
+    for (int i = 0; i < K * K; ++i) tp.w[i] = static_cast<float>(rq.taps[i]);
+    ParamsM p;
+    p.width = static_cast<int>(rq.width);
+    p.height = static_cast<int>(rq.height);
+    p.pitch = rq.pitch;
+    p.nstrips = static_cast<int>((rq.width + 16 * NT - 1) / (16 * NT));
+    int seg = rq.seg_rows;
+    if (seg <= 0) {
+        const long long target_waves = static_cast<long long>(rq.waves_per_simd) * 1024;
+        long long nsegs = (target_waves + p.nstrips - 1) / p.nstrips;
+        if (nsegs < 1) nsegs = 1;
+        seg = static_cast<int>((rq.height + nsegs - 1) / nsegs);
+        if (seg < 64) seg = 64;
+    }
+    seg = (seg + 15) / 16 * 16;
+    if (seg > (p.height + 15) / 16 * 16) seg = (p.height + 15) / 16 * 16;
+    p.seg_rows = seg;
+    const long long ntiles = static_cast<long long>(p.nstrips) * ((p.height + seg - 1) / seg);
+    if (ntiles > (1LL << 30)) return SSAM_E_UNSUPPORTED;
+    p.ntiles = static_cast<int>(ntiles);
+    p.nblocks = (p.ntiles + kWavesPerBlockM - 1) / kWavesPerBlockM;
+    hipLaunchKernelGGL((mfma2d_kernel<S, R, SPLIT, NT>), dim3(p.nblocks), dim3(kWave * kWavesPerBlockM), 0,
+                       st, tp, static_cast<const S*>(rq.src), static_cast<S*>(rq.dst), p);
+    return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
+}
+
+// Whether every tap is exact in the storage type (then one MFMA per row).
+template <typename S> bool taps_exact(const Req2D& rq) {
+    const int K = 2 * rq.radius + 1;
+    for (int i = 0; i < K * K; ++i) {
+        const float v = static_cast<float>(rq.taps[i]);
+        if (static_cast<double>(v) != rq.taps[i]) return false;
+        uint32_t u;
+        __builtin_memcpy(&u, &v, 4);
+        if constexpr (std::is_same<S, bf16_t>::value) {
+            if (u & 0xFFFFu) return false;  // more than 8 mantissa bits
+        } else {
+            const _Float16 h = static_cast<_Float16>(v);
+            if (static_cast<float>(h) != v) return false;
+        }
+    }
+    return true;
+}
+
+// Matrix-core form: BF16/FP16 storage, box/dense radius 3..8, one sweep, zero
+// boundary, rows 16-byte aligned.
+template <typename S> bool mfma_eligible(const Req2D& rq) {
+    if constexpr (!std::is_same<S, bf16_t>::value && !std::is_same<S, f16_t>::value) {
+        (void)rq;
+        return false;
+    } else {
+        const uintptr_t a = reinterpret_cast<uintptr_t>(rq.src) | reinterpret_cast<uintptr_t>(rq.dst);
+        return rq.shape == kBox && rq.steps == 1 && rq.boundary == kZero && rq.radius >= 3 &&
+               rq.radius <= 8 && rq.pitch % 8 == 0 && a % 16 == 0;
+    }
+}
+
+template <typename S> int launch_mfma(const Req2D& rq, hipStream_t st) {
+    if constexpr (!std::is_same<S, bf16_t>::value && !std::is_same<S, f16_t>::value) {
+        (void)rq;
+        (void)st;
+        return SSAM_E_UNSUPPORTED;
+    } else {
+        const bool split = !taps_exact<S>(rq);
+        switch (rq.radius) {
+#define SSAM_RM(r)                                                                              \
+    case r:                                                                                     \
+        return split ? launch_mfma_instance<S, r, true>(rq, st) : launch_mfma_instance<S, r, false>(rq, st);
+            SSAM_RM(3) SSAM_RM(4) SSAM_RM(5) SSAM_RM(6) SSAM_RM(7) SSAM_RM(8)
+#undef SSAM_RM
+            default:
+                return SSAM_E_UNSUPPORTED;
+        }
+    }
+}
+
+}  // namespace ssam

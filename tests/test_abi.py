@@ -144,7 +144,7 @@ def test_tuner_cache_io(tmp_path):
     assert lib.ssam_tune_load(None) == L.E_INVALID
     p = tmp_path / "tune.txt"
     assert lib.ssam_tune_save(str(p).encode()) == L.OK
-    assert p.read_text().startswith("# ssam autotune v1")
+    assert p.read_text().startswith("# ssam autotune v2")
     assert lib.ssam_tune_load(str(p).encode()) == L.E_INVALID   # header only: no entry
     p.write_text("0 0 1 1 8192 8192 8 0 0.125\n1 1 5 1 64 64 4 0 0.5\nbad line\n")
     assert lib.ssam_tune_load(str(p).encode()) == L.OK
@@ -182,4 +182,4 @@ def test_batch_plan_validates():
     assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
     assert lib.ssam_batch_run(None, None, 0) == L.E_INVALID
     assert lib.ssam_batch_plan_tune(None, None) == L.E_INVALID
-    assert lib.ssam_batch_plan_tiling(None, None, None, None) == L.E_INVALID
+    assert lib.ssam_batch_plan_tiling(None, None, None, None, None) == L.E_INVALID
