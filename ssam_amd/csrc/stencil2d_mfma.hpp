@@ -16,15 +16,19 @@
 // the VALU kernels, run down the rows.  The wave walks down a segment of rows in
 // steps of 16; each step loads one new 16-row block per tile, NB-2 steps ahead.
 //
-// Taps: FP32 from the kernel-argument segment, split into hi + lo storage-type
-// parts (two MFMAs per filter row) so the products carry ~16 mantissa bits of
-// each tap; inputs are exact in the storage type, accumulation is FP32.  Taps
-// that are exact in the storage type use one MFMA per row (SPLIT = false).
+// Taps: FP32 from the kernel-argument segment, converted to the storage type;
+// FP16 storage splits them into hi + lo parts (two MFMAs per filter row) so the
+// products carry ~22 mantissa bits of each tap; BF16 storage rounds its result
+// to 8 bits anyway and uses one part (SSAM_MFMA_SPLIT=1 splits it too); taps
+// exact in the storage type never split.  Inputs are exact in the storage type,
+// accumulation is FP32.
 //
 // Output D(i, j) sits in lane j + 16*(i/4), element i%4 (16x16 result layout);
 // rows at or past the segment end and columns past the width are not stored.
 // Zero boundary: rows and columns outside the grid read as 0.
 #pragma once
+#include <cstdlib>
+
 #include "ssam_device.hpp"
 #include "ssam_internal.hpp"
 #include "stencil2d.hpp"
@@ -59,6 +63,9 @@ struct ParamsM {
 };
 
 constexpr int kWavesPerBlockM = 4;
+
+// 16 zero bytes: the load target of input chunks outside the grid.
+static __device__ __align__(16) const uint32_t kZeroChunk[4] = {0u, 0u, 0u, 0u};
 
 // Rows shifted up by M inside each 16-lane row group: lane i gets x from lane
 // i+M while i+M < 16, else y from lane i+M-16 (x = rows b, y = rows b+1).
@@ -97,6 +104,9 @@ struct Mfma2D {
     };
 
     // Row block b (rows ys - R + 16 b + i) of tile t: 8 columns from x0+16t-8+8g.
+    // Edge tiles (CHK): rows and 8-column chunks wholly outside the grid read a
+    // zero chunk (one branch-free 16-byte load for every lane); only a chunk
+    // straddling the right edge loads its columns one by one.
     template <bool CHK>
     __device__ static __forceinline__ u32x4 load_block(const Lane& L, int b, int t) {
         const int r = L.ys - R + 16 * min(b, L.blast) + L.i;
@@ -105,22 +115,23 @@ struct Mfma2D {
         if constexpr (!CHK) {
             return gld<u32x4>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
         } else {
-            u32x4 v = {0u, 0u, 0u, 0u};
-            if (r >= 0 && r < L.p.height) {
-                if (c >= 0 && c + 8 <= L.p.width) {
-                    v = gld<u32x4>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
-                } else {
-                    uint32_t h[8];
+            const bool rok = r >= 0 && r < L.p.height;
+            const bool inside = c >= 0 && c + 8 <= L.p.width;
+            const void* a = (rok && inside)
+                                ? static_cast<const void*>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)))
+                                : static_cast<const void*>(kZeroChunk);
+            u32x4 v = gld<u32x4>(a);
+            if (rok && !inside && c < L.p.width && c + 8 > 0) {
+                uint32_t h[8];
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const int x = c + e;
-                        h[e] = (x >= 0 && x < L.p.width)
-                                   ? static_cast<uint32_t>(gld<uint16_t>(row + static_cast<long long>(x) * 2))
-                                   : 0u;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) v[q] = h[2 * q] | (h[2 * q + 1] << 16);
+                for (int e = 0; e < 8; ++e) {
+                    const int x = c + e;
+                    h[e] = (x >= 0 && x < L.p.width)
+                               ? static_cast<uint32_t>(gld<uint16_t>(row + static_cast<long long>(x) * 2))
+                               : 0u;
                 }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] = h[2 * q] | (h[2 * q + 1] << 16);
             }
             return v;
         }
@@ -178,34 +189,53 @@ struct Mfma2D {
         const int j = L.i;            // result layout: column = lane % 16
         const int rg = L.g;           // rows 4*rg .. 4*rg+3
         const int x = L.x0 + 16 * t + j;
+        // interior tiles: ye - ys is a whole number of 16-row steps, so every row
+        // is stored unconditionally (straight-line code keeps the memory counter
+        // waits exact, so prefetched row blocks stay in flight)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int y = y0 + 4 * rg + q;
             if (CHK && (y >= L.ye || x >= L.p.width)) continue;
-            if (!CHK && y >= L.ye) continue;
             char* p = L.dst + (static_cast<long long>(y) * L.p.pitch + x) * static_cast<long long>(sizeof(S));
             gst<S>(p, Storage<S>::store(d[q]));
         }
     }
 
+    // One 16-row step s (ring slot KT = s mod NB): prefetch block s+NB-1 into the
+    // slot of block s-1, then the MFMAs over blocks s and s+1, then the stores.
     template <bool CHK, int KT>
-    __device__ static __forceinline__ void step(const Lane& L, int s0, int nsteps, u32x4 (&ring)[NB][NT],
+    __device__ static __forceinline__ void step(const Lane& L, int s, u32x4 (&ring)[NB][NT],
                                                 const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) ring[(KT + NB - 1) % NB][t] = load_block<CHK>(L, s + NB - 1, t);
+        f32x4 D[NT], E[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) D[t] = E[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        rows_from<0>(D, E, ring[KT % NB], ring[(KT + 1) % NB], bh, bl);
+        const int y0 = L.ys + 16 * s;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) store_tile<CHK>(L, y0, t, SPLIT ? D[t] + E[t] : D[t]);
+    }
+
+    // a full ring period: NB steps, straight-line code
+    template <bool CHK, int KT>
+    __device__ static __forceinline__ void period(const Lane& L, int s0, u32x4 (&ring)[NB][NT],
+                                                  const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
         if constexpr (KT < NB) {
-            const int s = s0 + KT;
-            if (s < nsteps) {
-                // block s + NB - 1 into the slot of block s - 1
-#pragma unroll
-                for (int t = 0; t < NT; ++t) ring[(KT + NB - 1) % NB][t] = load_block<CHK>(L, s + NB - 1, t);
-                f32x4 D[NT], E[NT];
-#pragma unroll
-                for (int t = 0; t < NT; ++t) D[t] = E[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-                rows_from<0>(D, E, ring[KT % NB], ring[(KT + 1) % NB], bh, bl);
-                const int y0 = L.ys + 16 * s;
-#pragma unroll
-                for (int t = 0; t < NT; ++t) store_tile<CHK>(L, y0, t, SPLIT ? D[t] + E[t] : D[t]);
+            step<CHK, KT>(L, s0 + KT, ring, bh, bl);
+            period<CHK, KT + 1>(L, s0, ring, bh, bl);
+        }
+    }
+
+    // the last partial period: steps s0+KT < nsteps only
+    template <bool CHK, int KT>
+    __device__ static __forceinline__ void tail(const Lane& L, int s0, int nsteps, u32x4 (&ring)[NB][NT],
+                                                const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
+        if constexpr (KT < NB - 1) {
+            if (s0 + KT < nsteps) {
+                step<CHK, KT>(L, s0 + KT, ring, bh, bl);
+                tail<CHK, KT + 1>(L, s0, nsteps, ring, bh, bl);
             }
-            step<CHK, KT + 1>(L, s0, nsteps, ring, bh, bl);
         }
     }
 
@@ -217,7 +247,9 @@ struct Mfma2D {
 #pragma unroll
             for (int t = 0; t < NT; ++t) ring[b][t] = load_block<CHK>(L, b, t);
         const int nsteps = (L.ye - L.ys + 15) / 16;
-        for (int s0 = 0; s0 < nsteps; s0 += NB) step<CHK, 0>(L, s0, nsteps, ring, bh, bl);
+        int s0 = 0;
+        for (; s0 + NB <= nsteps; s0 += NB) period<CHK, 0>(L, s0, ring, bh, bl);
+        tail<CHK, 0>(L, s0, nsteps, ring, bh, bl);
     }
 
     __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const ParamsM& p,
@@ -240,7 +272,7 @@ struct Mfma2D {
         u32x4 bh[K], bl[K];
         build_taps(w, L.i, L.g, bh, bl);
         const bool interior = L.x0 - 8 >= 0 && L.x0 + OW + 8 <= p.width && L.ys - R >= 0 &&
-                              L.ys - R + 16 * L.blast + 15 < p.height;
+                              L.ys - R + 16 * L.blast + 15 < p.height && (L.ye - L.ys) % 16 == 0;
         if (interior)
             stream<false>(L, bh, bl);
         else
@@ -248,8 +280,9 @@ struct Mfma2D {
     }
 };
 
+// one set of taps leaves room for 3 waves per SIMD, split taps for 2
 template <typename S, int R, bool SPLIT, int NT>
-__global__ __launch_bounds__(kWave * kWavesPerBlockM) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(kWave * kWavesPerBlockM) __attribute__((amdgpu_waves_per_eu(SPLIT ? 2 : 3)))
 void mfma2d_kernel(Taps<float, (2 * R + 1) * (2 * R + 1)> tp, const S* __restrict__ src,
                    S* __restrict__ dst, ParamsM p) {
     (void)tp;
@@ -261,7 +294,7 @@ void mfma2d_kernel(Taps<float, (2 * R + 1) * (2 * R + 1)> tp, const S* __restric
 // outside what the matrix-core form covers.
 template <typename S, int R, bool SPLIT>
 int launch_mfma_instance(const Req2D& rq, hipStream_t st) {
-    constexpr int NT = SPLIT ? 2 : 4;  // tiles side by side per wave (register budget)
+    constexpr int NT = 2;  // tiles side by side per wave (register budget: 3 waves per SIMD)
     constexpr int K = 2 * R + 1;
     Taps<float, K * K> tp;
     for (int i = 0; i < K * K; ++i) tp.w[i] = static_cast<float>(rq.taps[i]);
@@ -278,7 +311,9 @@ int launch_mfma_instance(const Req2D& rq, hipStream_t st) {
         seg = static_cast<int>((rq.height + nsegs - 1) / nsegs);
         if (seg < 64) seg = 64;
     }
-    seg = (seg + 15) / 16 * 16;
+    // whole ring periods (4 steps of 16 rows) keep interior tiles on the
+    // straight-line loop
+    seg = (seg + 63) / 64 * 64;
     if (seg > (p.height + 15) / 16 * 16) seg = (p.height + 15) / 16 * 16;
     p.seg_rows = seg;
     const long long ntiles = static_cast<long long>(p.nstrips) * ((p.height + seg - 1) / seg);
@@ -321,13 +356,25 @@ template <typename S> bool mfma_eligible(const Req2D& rq) {
     }
 }
 
+// Taps split hi + lo unless they are exact in the storage type.  BF16 storage
+// rounds its result to 8 mantissa bits, so BF16 taps (another 2^-9 relative at
+// most) keep it well inside its error budget; SSAM_MFMA_SPLIT=1 forces the split.
+template <typename S> bool mfma_split(const Req2D& rq) {
+    if (taps_exact<S>(rq)) return false;
+    if (std::is_same<S, bf16_t>::value) {
+        const char* v = std::getenv("SSAM_MFMA_SPLIT");
+        return v && *v == '1';
+    }
+    return true;
+}
+
 template <typename S> int launch_mfma(const Req2D& rq, hipStream_t st) {
     if constexpr (!std::is_same<S, bf16_t>::value && !std::is_same<S, f16_t>::value) {
         (void)rq;
         (void)st;
         return SSAM_E_UNSUPPORTED;
     } else {
-        const bool split = !taps_exact<S>(rq);
+        const bool split = mfma_split<S>(rq);
         switch (rq.radius) {
 #define SSAM_RM(r)                                                                              \
     case r:                                                                                     \
