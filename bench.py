@@ -49,11 +49,71 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="skip FP64/BF16 side measurements")
     ap.add_argument("--only", default="", help="comma list of suite kernel names (profiling)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic tiling)")
+    ap.add_argument("--ingest-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU: turn two rocprofv3 --pmc runs of this bench (FETCH_SIZE, then "
+                         "WRITE_SIZE) into per-kernel HBM bytes under profiles/ and exit")
     return ap.parse_args()
+
+
+def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
+    """Per-launch HBM traffic of each suite kernel from rocprofv3 counter CSVs.
+
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what they read for
+    the 1 GiB stream copy of the same run (1 GiB read, 1 GiB written), so the
+    figures are bytes.  The last dispatch of each kernel instance is the timed
+    one (the autotuner's trial launches come first)."""
+    import csv
+    import re
+
+    def load(d):
+        f = next(pathlib.Path(d).rglob("*counter_collection.csv"))
+        out = {}
+        for r in csv.DictReader(open(f)):
+            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return out
+
+    fetch, write = load(fetch_dir), load(write_dir)
+    copy = next(k for k in fetch if "stream_copy" in k)
+    gib_kib = float(1 << 20)
+    fs = gib_kib / (sum(fetch[copy]) / len(fetch[copy]))
+    ws = gib_kib / (sum(write[copy]) / len(write[copy]))
+    names = {(1 if sh == "box" else 0, r, t): n for n, sh, r, t in SUITE}
+    alg = 2 * 4 * N * N
+    rows = []
+    for k in fetch:
+        m = re.search(r"ssam2d_kernel<float, (\d+), (\d+), \d+, (\d+),", k)
+        if not m or k not in write:
+            continue
+        r, shape, t = (int(v) for v in m.groups())
+        name = names.get((shape, r, t), f"{'box' if shape else 'star'}_r{r}_T{t}")
+        fb = fetch[k][-1] * fs * 1024
+        wb = write[k][-1] * ws * 1024
+        rec = dict(kernel=k.split("(")[0], fetch_bytes=round(fb), write_bytes=round(wb),
+                   hbm_bytes_per_launch=round(fb + wb), algorithmic_bytes=alg,
+                   traffic_over_algorithmic=round((fb + wb) / alg, 4),
+                   scale_fetch=round(fs, 4), scale_write=round(ws, 4))
+        (ROOT / "profiles" / f"traffic_{name}_8192_f32.json").write_text(json.dumps(rec, indent=1) + "\n")
+        rows.append((name, rec))
+    md = ["# HBM traffic per launch, FP32 suite kernels on 8192x8192 (MI355X)", "",
+          "Commands (two passes, counters in separate runs):", "",
+          "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline",
+          "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline",
+          "    python3 bench.py --ingest-pmc <F> <W>", "",
+          f"Calibration: the 1 GiB stream copy reads FETCH_SIZE x{fs:.3f} and WRITE_SIZE x{ws:.3f} "
+          "of its true bytes (KiB per dispatch); every figure below is scaled by these.", "",
+          "| kernel | fetch MiB | write MiB | HBM / algorithmic |", "|---|---|---|---|"]
+    for name, rec in sorted(rows):
+        md.append(f"| {name} | {rec['fetch_bytes'] / 2**20:.1f} | {rec['write_bytes'] / 2**20:.1f} | "
+                  f"{rec['traffic_over_algorithmic']:.3f} |")
+    (ROOT / "profiles" / "traffic_suite_8192_f32.md").write_text("\n".join(md) + "\n")
+    print("\n".join(md))
 
 
 def main():
     a = parse()
+    if a.ingest_pmc:
+        ingest_pmc(*a.ingest_pmc)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SSAM stencil path runs on a single MI355X here "
               "(multi-GPU slab decomposition is described in DESIGN.md, not run)", file=sys.stderr)
@@ -165,10 +225,11 @@ def main():
 
     extras = {}
     if not a.no_extras:
-        extras = side_runs(suite, taps, stream, per)
+        extras = side_runs(suite, taps, stream, per, tune=not a.no_tune)
         extras["config2_3d"] = side_3d(stream, copy_gbps)
-        extras["config3_conv"] = side_conv(stream, copy_gbps)
+        extras["config3_conv"] = side_conv(stream, copy_gbps, tune=not a.no_tune)
         extras["config4_batch"] = side_batch(stream)
+        extras["microbench_cdna4"] = {k: round(v, 3) for k, v in api.microbench(stream).items()}
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -205,9 +266,10 @@ def main():
     print(json.dumps(line))
 
 
-def side_runs(suite, taps, stream, per_f32):
-    """FP64 and BF16-storage versions of the suite (same input values), timed with
-    events; BF16 errors against the GPU FP64 results."""
+def side_runs(suite, taps, stream, per_f32, tune=True):
+    """FP64 and BF16/FP16-storage versions of the suite (same input values), each
+    kernel autotuned (tiling, fused steps per launch, VALU or matrix-core form) and
+    timed with events; BF16/FP16 errors against the GPU FP64 results."""
     import numpy as np
     import torch
 
@@ -227,6 +289,8 @@ def side_runs(suite, taps, stream, per_f32):
         for i, (name, shape, r, t) in enumerate(suite):
             f = taps(shape, r)
             sh = L.STAR if shape == "star" else L.BOX
+            if tune:
+                api.tune2d(src, dst, f, shape=sh, steps=t, work=work, stream=stream)
             api.stencil2d(src, dst, f, shape=sh, steps=t, work=work, stream=stream)  # warm
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
@@ -308,7 +372,7 @@ def side_3d(stream, copy_gbps):
     return out
 
 
-def side_conv(stream, copy_gbps):
+def side_conv(stream, copy_gbps, tune=True):
     """8192^2 image U[0,1), K x K taps U[-1,1)/K^2: FP32 plain vs BF16 / FP16 storage
     (FP32 accumulation); errors against the FP64 GPU path on the same values."""
     import torch
@@ -330,6 +394,8 @@ def side_conv(stream, copy_gbps):
                           ("fp16", torch.float16)):
             src = x64 if dt == torch.float64 else x64.to(dt)
             dst = torch.empty_like(src)
+            if tune:
+                api.tune2d(src, dst, f, shape=L.DENSE, stream=stream)
             ms = _time(lambda: api.stencil2d(src, dst, f, shape=L.DENSE, stream=stream), stream)
             ent = dict(ms=round(ms, 4), gpts=round(n * n / ms / 1e6, 2))
             if dt != torch.float64:

@@ -241,6 +241,18 @@ int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* 
                            int32_t* zseg_3d, int32_t* affinity);
 void ssam_batch_plan_destroy(ssam_batch_plan plan);
 
+/* ---------------------------------------------------------------- micro-benchmarks
+ * CDNA4 measurements behind the kernels' design (counterpart of the reference's
+ * shuffle / shared-memory / FMA latency measurements, SSAM paper Table 2):
+ *   which 0..3: latency in cycles of one dependent operation on one wave --
+ *     0 FP32 FMA, 1 DPP wave_shr:1 fused into v_add_f32 (the systolic hop),
+ *     2 ds_bpermute + add, 3 LDS store + load round trip + add;
+ *   which 4..8: wave-instructions per cycle per CU over the whole device, 8 waves
+ *     per CU with 8 independent chains each -- 4 FMA, 5 DPP add, 6 ds_bpermute,
+ *     7 LDS round trip, 8 v_mfma_f32_16x16x32_bf16.
+ * Cycles are shader clocks (s_memtime).  Synchronises `stream`. */
+int ssam_microbench(int which, double* result, void* stream);
+
 /* ---------------------------------------------------------------- autotuner
  * Times candidate tilings of one 2D request on the caller's buffers (dst and work
  * are overwritten; the stream is synchronised): tile height, as the number of

@@ -113,6 +113,7 @@ _PROTOS = {
                                   ctypes.POINTER(_i64)]),
     "ssam_batch_plan_destroy": (None, [ctypes.c_void_p]),
     "ssam_batch_plan_tune": (_i, [ctypes.c_void_p, _vp]),
+    "ssam_microbench": (_i, [_i, ctypes.POINTER(ctypes.c_double), _vp]),
     "ssam_batch_plan_tiling": (_i, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(ctypes.c_int32)]),

@@ -319,3 +319,18 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+MICROBENCH = ("fma_latency_cycles", "dpp_add_latency_cycles", "bpermute_latency_cycles",
+              "lds_roundtrip_latency_cycles", "fma_per_cycle_per_cu", "dpp_add_per_cycle_per_cu",
+              "bpermute_per_cycle_per_cu", "lds_roundtrip_per_cycle_per_cu", "mfma_bf16_16x16x32_per_cycle_per_cu")
+
+
+def microbench(stream: Optional[torch.cuda.Stream] = None) -> dict:
+    """All CDNA4 micro-benchmarks of ssam_microbench, by name."""
+    out = {}
+    for i, name in enumerate(MICROBENCH):
+        v = ctypes.c_double()
+        check(L.lib().ssam_microbench(i, ctypes.byref(v), _stream(stream)), "ssam_microbench")
+        out[name] = v.value
+    return out

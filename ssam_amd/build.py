@@ -43,6 +43,7 @@ SOURCES = [
     "stencil2d_f16.hip",
     "stencil3d.hip",
     "batch.hip",
+    "microbench.hip",
 ]
 
 

@@ -40,8 +40,9 @@ namespace ssam {
 // ------------------------------------------------------------------ 2D menu
 // Shapes the persistent 2D kernel runs in one launch (mixes of 5-point stars and
 // 3x3 / 5x5 / 11x11 boxes); other shapes fall back to their own launch.
-enum Kind2D : int { k2Star1 = 0, k2Box1 = 1, k2Box2 = 2, k2Box5 = 3, k2None = -1 };
-constexpr int kNumKind2D = 4;
+// k2Box5M: the 11x11 box on the matrix cores (BF16/FP16 storage).
+enum Kind2D : int { k2Star1 = 0, k2Box1 = 1, k2Box2 = 2, k2Box5 = 3, k2Box5M = 4, k2None = -1 };
+constexpr int kNumKind2D = 4;  // tile-height slots (k2Box5M uses k2Box5's)
 
 inline int kind2d(int shape, int r) {
     if (shape == kStar && r == 1) return k2Star1;
@@ -62,6 +63,7 @@ struct BatchItem2D {
     const void* src;
     void* dst;
     Params2D p;
+    ParamsM pm;    // k2Box5M tiles
     int kind;
     int taps_off;  // offset into the taps array, in accumulation-type elements
 };
@@ -133,6 +135,10 @@ void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restric
             case k2Star1: Inst2D<S, 1, kStar>::K::run_tile(src, dst, b.p, w, strip, seg); break;
             case k2Box1: Inst2D<S, 1, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
             case k2Box2: Inst2D<S, 2, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
+            case k2Box5M:
+                if constexpr (std::is_same<S, bf16_t>::value || std::is_same<S, f16_t>::value)
+                    Mfma2D<S, 5, std::is_same<S, f16_t>::value, 2, 1>::run_tile(src, dst, b.pm, w, strip, seg);
+                break;
             default: Inst2D<S, 5, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
         }
     });
@@ -367,6 +373,28 @@ void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p,
     }
 }
 
+// Tiles of the matrix-core 11x11 box: 32-column strips, rows in whole ring
+// periods (64).
+void tiles_for_mfma(const ssam_batch_item& it, int idx, int seg_rows, ParamsM* p, ItemTiles* out) {
+    constexpr int OW = 32;
+    p->width = static_cast<int>(it.nx);
+    p->height = static_cast<int>(it.ny);
+    p->pitch = it.pitch;
+    p->nstrips = static_cast<int>((it.nx + OW - 1) / OW);
+    p->seg_rows = std::max(64, (seg_rows + 63) / 64 * 64);
+    const int nsegs = (p->height + p->seg_rows - 1) / p->seg_rows;
+    p->ntiles = p->nstrips * nsegs;
+    p->nblocks = 0;
+    for (int sg = 0; sg < nsegs; ++sg) {
+        const int rows = std::min(p->seg_rows, p->height - sg * p->seg_rows);
+        for (int st = 0; st < p->nstrips; ++st) {
+            out->tiles.push_back(int4{idx, st, sg, 0});
+            // a quarter of the VALU kernel's cost per point (measured ratio)
+            out->cost.push_back(static_cast<double>(rows + 10) * (121.0 + 8.0) * 0.25 * OW / 248.0);
+        }
+    }
+}
+
 void free_group(Plan::Group& g) {
     if (g.d_items) (void)hipFree(g.d_items);
     if (g.d_tiles) (void)hipFree(g.d_tiles);
@@ -398,6 +426,13 @@ int build_group2d(Plan* P, Plan::Group& g) {
             }
         const int idx = static_cast<int>(k);
         const int seg = P->seg2d[b.kind];
+        // 11x11 boxes in BF16/FP16 go to the matrix cores when rows are 16-byte aligned
+        if (b.kind == k2Box5 && (std::is_same<S, bf16_t>::value || std::is_same<S, f16_t>::value) &&
+            it.pitch % 8 == 0 && ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 16) == 0) {
+            b.kind = k2Box5M;
+            tiles_for_mfma(it, idx, seg, &b.pm, &per[k]);
+            continue;
+        }
         switch (b.kind) {
             case k2Star1: tiles_for_2d<S, 1, kStar>(it, idx, seg, &b.p, &per[k]); break;
             case k2Box1: tiles_for_2d<S, 1, kBox>(it, idx, seg, &b.p, &per[k]); break;

@@ -116,7 +116,11 @@ template <typename S>
 int dispatch2d(const Req2D& rq, hipStream_t st) {
     constexpr int V = Lanes2D<S>::V, VT1 = Lanes2D<S>::VT1, VT2 = Lanes2D<S>::VT2;
     if (rq.steps == 1) {
-        if (rq.form != kFormValu && mfma_eligible<S>(rq)) return launch_mfma<S>(rq, st);
+        // matrix cores: by default from radius 2 (where they beat the VALU kernels
+        // on MI355X); the autotuner may pick them for radius 1 too
+        if (mfma_eligible<S>(rq) &&
+            (rq.form == kFormMfma || (rq.form == kFormAuto && rq.radius >= 2)))
+            return launch_mfma<S>(rq, st);
         if (rq.shape == kStar) return dispatch_radius<S, kStar, V, 1>(rq, st);
         // FP64 storage accumulates in FP64: compensation changes nothing there
         if (rq.shape == kBoxKahan && sizeof(S) != 8) return dispatch_radius<S, kBoxKahan, V, 1>(rq, st);

@@ -6,8 +6,9 @@
 //     D(i,j) += sum_k A_m(i,k) B_m(k,j),   A_m(i,k) = in(y0+i+m-r, x0-8+k),
 //     B_m(k,j) = w[m][k-8+r-j]  (0 <= k-8+r-j <= 2r, else 0),  k = 0..31,
 // so one MFMA per filter row covers the 16 + 2r <= 32 input columns of the tile
-// (r <= 8).  The input window starts 8 columns left of the tile, so every lane's
-// A operand (8 consecutive columns of one row) is one aligned 16-byte load.
+// (r <= 8; r = 9 takes two 32-column windows).  The input window starts 8 (16)
+// columns left of the tile, so every lane's A operand (8 consecutive columns of
+// one row) is one aligned 16-byte load.
 //
 // Register cache instead of LDS: lane (i, g) of a 16-lane row group holds, for
 // each of the wave's NT tiles side by side, input rows as 16-row blocks in a
@@ -85,12 +86,17 @@ template <int M> __device__ __forceinline__ uint32_t shift_rows(uint32_t x, uint
     }
 }
 
-template <typename S, int R, bool SPLIT, int NT>
+// NC 32-column windows per 16-column tile: 1 for r <= 8, 2 for r = 9 (the window
+// then starts 16 columns left of the tile).
+template <typename S, int R, bool SPLIT, int NT, int NC>
 struct Mfma2D {
-    static_assert(R >= 1 && R <= 8, "one 32-column window per 16-column tile");
+    static_assert(R >= 1 && 16 + 2 * R <= 32 * NC && R <= 8 * NC, "window must hold 16 + 2r columns");
+    static_assert(2 * R <= 32, "rows s..s+2 of the ring hold the 16 + 2r rows of a step");
     static constexpr int K = 2 * R + 1;
     static constexpr int NB = 4;        // row blocks in the ring (two used + two ahead)
     static constexpr int OW = 16 * NT;  // output columns per strip
+    static constexpr int XS = 8 * NC;   // window start, columns left of the tile
+    static constexpr int NA = NT * NC;  // A operands per row block
     using Op = MfmaOp<S>;
 
     struct Lane {
@@ -103,14 +109,15 @@ struct Mfma2D {
         int blast;     // last row block any output row needs
     };
 
-    // Row block b (rows ys - R + 16 b + i) of tile t: 8 columns from x0+16t-8+8g.
+    // Row block b (rows ys - R + 16 b + i) of operand a = t*NC + c (tile t, window
+    // c): 8 columns from x0 + 16t - XS + 32c + 8g.
     // Edge tiles (CHK): rows and 8-column chunks wholly outside the grid read a
     // zero chunk (one branch-free 16-byte load for every lane); only a chunk
     // straddling the right edge loads its columns one by one.
     template <bool CHK>
-    __device__ static __forceinline__ u32x4 load_block(const Lane& L, int b, int t) {
+    __device__ static __forceinline__ u32x4 load_block(const Lane& L, int b, int a) {
         const int r = L.ys - R + 16 * min(b, L.blast) + L.i;
-        const int c = L.x0 + 16 * t - 8 + 8 * L.g;
+        const int c = L.x0 + 16 * (a / NC) - XS + 32 * (a % NC) + 8 * L.g;
         const char* row = L.src + static_cast<long long>(r) * L.p.pitch * static_cast<long long>(sizeof(S));
         if constexpr (!CHK) {
             return gld<u32x4>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
@@ -137,15 +144,16 @@ struct Mfma2D {
         }
     }
 
-    // B operand of filter row m (hi or lo part), built once per wave.
+    // B operand of filter row m, window c (hi or lo part), built once per wave.
     __device__ static __forceinline__ void build_taps(const float* __restrict__ w, int j, int g,
-                                                      u32x4 (&bh)[K], u32x4 (&bl)[K]) {
+                                                      u32x4 (&bh)[K * NC], u32x4 (&bl)[K * NC]) {
 #pragma unroll
-        for (int m = 0; m < K; ++m) {
+        for (int mc = 0; mc < K * NC; ++mc) {
+            const int m = mc / NC, c = mc % NC;
             uint32_t h[8], l[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int n = 8 * g + e - 8 + R - j;
+                const int n = 32 * c + 8 * g + e - XS + R - j;
                 const float v = (n >= 0 && n <= 2 * R) ? w[m * K + n] : 0.f;
                 const S hi = Storage<S>::store(v);
                 const S lo = Storage<S>::store(v - Storage<S>::load(hi));
@@ -154,8 +162,8 @@ struct Mfma2D {
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                bh[m][q] = h[2 * q] | (h[2 * q + 1] << 16);
-                bl[m][q] = l[2 * q] | (l[2 * q + 1] << 16);
+                bh[mc][q] = h[2 * q] | (h[2 * q + 1] << 16);
+                bl[mc][q] = l[2 * q] | (l[2 * q + 1] << 16);
             }
         }
     }
@@ -169,18 +177,23 @@ struct Mfma2D {
     }
 
     // hi and lo parts accumulate separately (independent MFMA chains)
+    // X, Y, Z: row blocks s, s+1, s+2 (Z is read only when 2r > 16)
     template <int M>
-    __device__ static __forceinline__ void rows_from(f32x4 (&D)[NT], f32x4 (&E)[NT], const u32x4 (&X)[NT],
-                                                     const u32x4 (&Y)[NT], const u32x4 (&bh)[K],
-                                                     const u32x4 (&bl)[K]) {
+    __device__ static __forceinline__ void rows_from(f32x4 (&D)[NT], f32x4 (&E)[NT], const u32x4 (&X)[NA],
+                                                     const u32x4 (&Y)[NA], const u32x4 (&Z)[NA],
+                                                     const u32x4 (&bh)[K * NC], const u32x4 (&bl)[K * NC]) {
         if constexpr (M < K) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const u32x4 a = shifted<M>(X[t], Y[t]);
-                D[t] = Op::mma(a, bh[M], D[t]);
-                if constexpr (SPLIT) E[t] = Op::mma(a, bl[M], E[t]);
-            }
-            rows_from<M + 1>(D, E, X, Y, bh, bl);
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const int q = t * NC + c;
+                    const u32x4 a = M <= 16 ? shifted<(M <= 16 ? M : 0)>(X[q], Y[q])
+                                            : shifted<(M > 16 ? M - 16 : 0)>(Y[q], Z[q]);
+                    D[t] = Op::mma(a, bh[M * NC + c], D[t]);
+                    if constexpr (SPLIT) E[t] = Op::mma(a, bl[M * NC + c], E[t]);
+                }
+            rows_from<M + 1>(D, E, X, Y, Z, bh, bl);
         }
     }
 
@@ -204,14 +217,14 @@ struct Mfma2D {
     // One 16-row step s (ring slot KT = s mod NB): prefetch block s+NB-1 into the
     // slot of block s-1, then the MFMAs over blocks s and s+1, then the stores.
     template <bool CHK, int KT>
-    __device__ static __forceinline__ void step(const Lane& L, int s, u32x4 (&ring)[NB][NT],
-                                                const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
+    __device__ static __forceinline__ void step(const Lane& L, int s, u32x4 (&ring)[NB][NA],
+                                                const u32x4 (&bh)[K * NC], const u32x4 (&bl)[K * NC]) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) ring[(KT + NB - 1) % NB][t] = load_block<CHK>(L, s + NB - 1, t);
+        for (int a = 0; a < NA; ++a) ring[(KT + NB - 1) % NB][a] = load_block<CHK>(L, s + NB - 1, a);
         f32x4 D[NT], E[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) D[t] = E[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        rows_from<0>(D, E, ring[KT % NB], ring[(KT + 1) % NB], bh, bl);
+        rows_from<0>(D, E, ring[KT % NB], ring[(KT + 1) % NB], ring[(KT + 2) % NB], bh, bl);
         const int y0 = L.ys + 16 * s;
 #pragma unroll
         for (int t = 0; t < NT; ++t) store_tile<CHK>(L, y0, t, SPLIT ? D[t] + E[t] : D[t]);
@@ -219,8 +232,8 @@ struct Mfma2D {
 
     // a full ring period: NB steps, straight-line code
     template <bool CHK, int KT>
-    __device__ static __forceinline__ void period(const Lane& L, int s0, u32x4 (&ring)[NB][NT],
-                                                  const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
+    __device__ static __forceinline__ void period(const Lane& L, int s0, u32x4 (&ring)[NB][NA],
+                                                  const u32x4 (&bh)[K * NC], const u32x4 (&bl)[K * NC]) {
         if constexpr (KT < NB) {
             step<CHK, KT>(L, s0 + KT, ring, bh, bl);
             period<CHK, KT + 1>(L, s0, ring, bh, bl);
@@ -229,8 +242,8 @@ struct Mfma2D {
 
     // the last partial period: steps s0+KT < nsteps only
     template <bool CHK, int KT>
-    __device__ static __forceinline__ void tail(const Lane& L, int s0, int nsteps, u32x4 (&ring)[NB][NT],
-                                                const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
+    __device__ static __forceinline__ void tail(const Lane& L, int s0, int nsteps, u32x4 (&ring)[NB][NA],
+                                                const u32x4 (&bh)[K * NC], const u32x4 (&bl)[K * NC]) {
         if constexpr (KT < NB - 1) {
             if (s0 + KT < nsteps) {
                 step<CHK, KT>(L, s0 + KT, ring, bh, bl);
@@ -240,23 +253,24 @@ struct Mfma2D {
     }
 
     template <bool CHK>
-    __device__ static __forceinline__ void stream(const Lane& L, const u32x4 (&bh)[K], const u32x4 (&bl)[K]) {
-        u32x4 ring[NB][NT];
+    __device__ static __forceinline__ void stream(const Lane& L, const u32x4 (&bh)[K * NC],
+                                                  const u32x4 (&bl)[K * NC]) {
+        u32x4 ring[NB][NA];
 #pragma unroll
         for (int b = 0; b < NB - 1; ++b)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) ring[b][t] = load_block<CHK>(L, b, t);
+            for (int a = 0; a < NA; ++a) ring[b][a] = load_block<CHK>(L, b, a);
         const int nsteps = (L.ye - L.ys + 15) / 16;
         int s0 = 0;
         for (; s0 + NB <= nsteps; s0 += NB) period<CHK, 0>(L, s0, ring, bh, bl);
         tail<CHK, 0>(L, s0, nsteps, ring, bh, bl);
     }
 
-    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const ParamsM& p,
-                               const float* __restrict__ w) {
-        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-        const int tile = xcd_remap(blockIdx.x, p.nblocks) * kWavesPerBlockM + wv;
-        if (tile >= p.ntiles) return;  // whole wave leaves; no barriers are used
+    // One tile: strip `strip` (OW output columns), output rows [seg*seg_rows, ...)
+    // of the grid described by p (wave-uniform arguments).
+    __device__ static __forceinline__ void run_tile(const S* __restrict__ src, S* __restrict__ dst,
+                                                    const ParamsM& p, const float* __restrict__ w,
+                                                    int strip, int seg) {
         const int lane = threadIdx.x & (kWave - 1);
         Lane L;
         L.src = reinterpret_cast<const char*>(src);
@@ -264,37 +278,48 @@ struct Mfma2D {
         L.p = p;
         L.i = lane & 15;
         L.g = lane >> 4;
-        L.x0 = (tile % p.nstrips) * OW;
-        L.ys = (tile / p.nstrips) * p.seg_rows;
+        L.x0 = strip * OW;
+        L.ys = seg * p.seg_rows;
         L.ye = min(p.height, L.ys + p.seg_rows);
         // output rows up to ye-1 need input rows up to ye-1+R: block index of that row
         L.blast = (L.ye - 1 + R - (L.ys - R)) / 16;
-        u32x4 bh[K], bl[K];
+        u32x4 bh[K * NC], bl[K * NC];
         build_taps(w, L.i, L.g, bh, bl);
-        const bool interior = L.x0 - 8 >= 0 && L.x0 + OW + 8 <= p.width && L.ys - R >= 0 &&
+        const bool interior = L.x0 - XS >= 0 && L.x0 + OW - 16 - XS + 32 * NC <= p.width && L.ys - R >= 0 &&
                               L.ys - R + 16 * L.blast + 15 < p.height && (L.ye - L.ys) % 16 == 0;
         if (interior)
             stream<false>(L, bh, bl);
         else
             stream<true>(L, bh, bl);
     }
+
+    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const ParamsM& p,
+                               const float* __restrict__ w) {
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+        const int tile = xcd_remap(blockIdx.x, p.nblocks) * kWavesPerBlockM + wv;
+        if (tile >= p.ntiles) return;  // whole wave leaves; no barriers are used
+        run_tile(src, dst, p, w, tile % p.nstrips, tile / p.nstrips);
+    }
 };
 
-// one set of taps leaves room for 3 waves per SIMD, split taps for 2
-template <typename S, int R, bool SPLIT, int NT>
-__global__ __launch_bounds__(kWave * kWavesPerBlockM) __attribute__((amdgpu_waves_per_eu(SPLIT ? 2 : 3)))
+// one set of taps in one window leaves room for 3 waves per SIMD, split taps for
+// 2; two windows (radius 9) need the whole register file (1)
+template <typename S, int R, bool SPLIT, int NT, int NC>
+__global__ __launch_bounds__(kWave * kWavesPerBlockM)
+__attribute__((amdgpu_waves_per_eu(NC > 1 ? 1 : (SPLIT ? 2 : 3))))
 void mfma2d_kernel(Taps<float, (2 * R + 1) * (2 * R + 1)> tp, const S* __restrict__ src,
                    S* __restrict__ dst, ParamsM p) {
     (void)tp;
     const float* __restrict__ w = (const float*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Mfma2D<S, R, SPLIT, NT>::run(src, dst, p, w);
+    Mfma2D<S, R, SPLIT, NT, NC>::run(src, dst, p, w);
 }
 
 // Host launch for one request; returns SSAM_E_UNSUPPORTED when the request is
 // outside what the matrix-core form covers.
 template <typename S, int R, bool SPLIT>
 int launch_mfma_instance(const Req2D& rq, hipStream_t st) {
-    constexpr int NT = 2;  // tiles side by side per wave (register budget: 3 waves per SIMD)
+    constexpr int NC = R <= 8 ? 1 : 2;  // 32-column windows per tile
+    constexpr int NT = 3 - NC;          // tiles side by side per wave (register budget)
     constexpr int K = 2 * R + 1;
     Taps<float, K * K> tp;
     for (int i = 0; i < K * K; ++i) tp.w[i] = static_cast<float>(rq.taps[i]);
@@ -320,7 +345,7 @@ int launch_mfma_instance(const Req2D& rq, hipStream_t st) {
     if (ntiles > (1LL << 30)) return SSAM_E_UNSUPPORTED;
     p.ntiles = static_cast<int>(ntiles);
     p.nblocks = (p.ntiles + kWavesPerBlockM - 1) / kWavesPerBlockM;
-    hipLaunchKernelGGL((mfma2d_kernel<S, R, SPLIT, NT>), dim3(p.nblocks), dim3(kWave * kWavesPerBlockM), 0,
+    hipLaunchKernelGGL((mfma2d_kernel<S, R, SPLIT, NT, NC>), dim3(p.nblocks), dim3(kWave * kWavesPerBlockM), 0,
                        st, tp, static_cast<const S*>(rq.src), static_cast<S*>(rq.dst), p);
     return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
 }
@@ -343,16 +368,19 @@ template <typename S> bool taps_exact(const Req2D& rq) {
     return true;
 }
 
-// Matrix-core form: BF16/FP16 storage, box/dense radius 3..8, one sweep, zero
-// boundary, rows 16-byte aligned.
+// Matrix-core form: BF16/FP16 storage, box/dense radius 1..8 (BF16: 9), one
+// sweep, zero boundary, rows 16-byte aligned.
 template <typename S> bool mfma_eligible(const Req2D& rq) {
     if constexpr (!std::is_same<S, bf16_t>::value && !std::is_same<S, f16_t>::value) {
         (void)rq;
         return false;
     } else {
+        // radius 9 needs two windows per tile; FP16 there would also need split
+        // taps (past the register budget), so only BF16 takes it
         const uintptr_t a = reinterpret_cast<uintptr_t>(rq.src) | reinterpret_cast<uintptr_t>(rq.dst);
-        return rq.shape == kBox && rq.steps == 1 && rq.boundary == kZero && rq.radius >= 3 &&
-               rq.radius <= 8 && rq.pitch % 8 == 0 && a % 16 == 0;
+        const int rmax = std::is_same<S, bf16_t>::value ? 9 : 8;
+        return rq.shape == kBox && rq.steps == 1 && rq.boundary == kZero && rq.radius >= 1 &&
+               rq.radius <= rmax && rq.pitch % 8 == 0 && a % 16 == 0;
     }
 }
 
@@ -360,7 +388,7 @@ template <typename S> bool mfma_eligible(const Req2D& rq) {
 // rounds its result to 8 mantissa bits, so BF16 taps (another 2^-9 relative at
 // most) keep it well inside its error budget; SSAM_MFMA_SPLIT=1 forces the split.
 template <typename S> bool mfma_split(const Req2D& rq) {
-    if (taps_exact<S>(rq)) return false;
+    if (taps_exact<S>(rq) || rq.radius > 8) return false;
     if (std::is_same<S, bf16_t>::value) {
         const char* v = std::getenv("SSAM_MFMA_SPLIT");
         return v && *v == '1';
@@ -379,8 +407,10 @@ template <typename S> int launch_mfma(const Req2D& rq, hipStream_t st) {
 #define SSAM_RM(r)                                                                              \
     case r:                                                                                     \
         return split ? launch_mfma_instance<S, r, true>(rq, st) : launch_mfma_instance<S, r, false>(rq, st);
-            SSAM_RM(3) SSAM_RM(4) SSAM_RM(5) SSAM_RM(6) SSAM_RM(7) SSAM_RM(8)
+            SSAM_RM(1) SSAM_RM(2) SSAM_RM(3) SSAM_RM(4) SSAM_RM(5) SSAM_RM(6) SSAM_RM(7) SSAM_RM(8)
 #undef SSAM_RM
+            case 9:
+                return launch_mfma_instance<S, 9, false>(rq, st);
             default:
                 return SSAM_E_UNSUPPORTED;
         }

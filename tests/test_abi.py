@@ -183,3 +183,10 @@ def test_batch_plan_validates():
     assert lib.ssam_batch_run(None, None, 0) == L.E_INVALID
     assert lib.ssam_batch_plan_tune(None, None) == L.E_INVALID
     assert lib.ssam_batch_plan_tiling(None, None, None, None, None) == L.E_INVALID
+
+
+def test_microbench_validates():
+    v = ctypes.c_double()
+    assert L.lib().ssam_microbench(-1, ctypes.byref(v), None) == L.E_INVALID
+    assert L.lib().ssam_microbench(9, ctypes.byref(v), None) == L.E_INVALID
+    assert L.lib().ssam_microbench(0, None, None) == L.E_INVALID

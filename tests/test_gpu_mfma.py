@@ -33,7 +33,7 @@ def form(request, monkeypatch):
     return request.param
 
 
-@pytest.mark.parametrize("r", range(3, 9))
+@pytest.mark.parametrize("r", range(1, 10))
 @pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
 # pitches are multiples of 8 (the matrix-core form's row alignment) except the
 # last, which runs the VALU kernels under either setting
@@ -49,7 +49,7 @@ def test_seeded_box(form, r, storage, h, w, pad):
     assert e["max_rel"] <= TOL[storage], (form, e)
 
 
-@pytest.mark.parametrize("k", [7, 9, 15, 17])
+@pytest.mark.parametrize("k", [3, 5, 7, 9, 15, 17, 19])
 @pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
 def test_conv_taps(form, k, storage):
     """The convolution benchmark's taps (U[-1,1) / K^2) on a U[0,1) image."""
@@ -63,7 +63,7 @@ def test_conv_taps(form, k, storage):
     assert e["max_rel"] <= TOL[storage], (form, e)
 
 
-@pytest.mark.parametrize("r", [3, 5, 8])
+@pytest.mark.parametrize("r", [1, 2, 3, 5, 8, 9])
 @pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
 def test_small_integers_exact(form, r, storage):
     """Inputs and taps in {0, 1}: every product and sum is exact, so the result
