@@ -49,10 +49,57 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="skip FP64/BF16 side measurements")
     ap.add_argument("--only", default="", help="comma list of suite kernel names (profiling)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic tiling)")
+    ap.add_argument("--ingest-trace", metavar="TRACE_DIR",
+                    help="no GPU: summarise a rocprofv3 --kernel-trace run of this bench (the "
+                         "timed dispatches of each suite kernel) into profiles/ and exit")
     ap.add_argument("--ingest-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="no GPU: turn two rocprofv3 --pmc runs of this bench (FETCH_SIZE, then "
                          "WRITE_SIZE) into per-kernel HBM bytes under profiles/ and exit")
     return ap.parse_args()
+
+
+def ingest_trace(trace_dir: str, steps: int, warmup: int) -> None:
+    """Per-kernel duration of the suite's warm-up and timed dispatches from a
+    rocprofv3 kernel trace of `bench.py --steps S --warmup W --no-extras
+    --no-cpu-baseline` (the dispatches between the 16-byte marker copy and the
+    bandwidth copy)."""
+    import csv
+
+    f = next(pathlib.Path(trace_dir).rglob("*kernel_trace.csv"))
+    recs = []
+    for r in csv.DictReader(open(f)):
+        grid = int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
+        recs.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], grid,
+                     r.get("VGPR_Count", ""), r.get("Accum_VGPR_Count", "")))
+    recs.sort()
+    # the timed phase: after the 16-byte marker copy, up to the bandwidth copy
+    t0 = next(x[0] for x in recs if "stream_copy" in x[2] and x[3] <= 256)
+    t1 = next((x[0] for x in recs if "stream_copy" in x[2] and x[3] > 256 and x[0] > t0), recs[-1][1] + 1)
+    runs = {}
+    for st, en, k, grid, vg, ag in recs:
+        if t0 < st < t1 or "stream_copy" in k:
+            runs.setdefault(k, []).append(((en - st) / 1000.0, vg, ag, grid))
+    rows = []
+    for k, v in runs.items():
+        d = [x[0] for x in v]
+        rows.append((sum(d) / len(d), k, len(d), min(d), max(d), v[-1][1], v[-1][2], v[-1][3]))
+    rows.sort(reverse=True)
+    md = ["# rocprofv3 kernel trace: 2D SSAM suite, 8192x8192 FP32 (bench workload)", "",
+          "Command (one MI355X, repo root):", "",
+          f"    rocprofv3 --kernel-trace --stats --output-format csv -d <D> -o run -- python3 bench.py "
+          f"--steps {steps} --warmup {warmup} --no-extras --no-cpu-baseline",
+          f"    python3 bench.py --ingest-trace <D> --steps {steps} --warmup {warmup}", "",
+          f"Rows average the {warmup} warm-up + {steps} timed steps' dispatches (after the bench's "
+          "16-byte marker copy, i.e. after the autotuner's candidates) and the stream copies; "
+          "durations in microseconds.", "",
+          "| kernel | dispatches | avg us | min us | max us | arch VGPR | acc VGPR | grid threads |",
+          "|---|---|---|---|---|---|---|---|"]
+    for avg, k, n, mn, mx, vg, ag, gr in rows:
+        md.append(f"| `{k.split('(')[0]}` | {n} | {avg:.1f} | {mn:.1f} | {mx:.1f} | {vg} | {ag} | {gr} |")
+    md += ["", "Template parameters: ssam2d_kernel<storage, radius, shape (0 star, 1 box), columns per "
+           "lane V, fused steps T, prefetch rows PF, ring form, ticks per block U, min waves/EU>."]
+    (ROOT / "profiles" / "suite_8192_f32_kernel_trace.md").write_text("\n".join(md) + "\n")
+    print("\n".join(md))
 
 
 def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
@@ -60,24 +107,30 @@ def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
 
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what they read for
     the 1 GiB stream copy of the same run (1 GiB read, 1 GiB written), so the
-    figures are bytes.  The last dispatch of each kernel instance is the timed
-    one (the autotuner's trial launches come first)."""
+    figures are bytes.  Only the dispatches after the bench's 16-byte marker copy
+    count (the autotuner's trial launches come before it)."""
     import csv
     import re
 
     def load(d):
         f = next(pathlib.Path(d).rglob("*counter_collection.csv"))
-        out = {}
-        for r in csv.DictReader(open(f)):
-            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
-        return out
+        rows = sorted(csv.DictReader(open(f)), key=lambda r: int(r["Dispatch_Id"]))
+        # dispatches after the 16-byte marker copy are the timed suite launches
+        mark = next((int(r["Dispatch_Id"]) for r in rows
+                     if "stream_copy" in r["Kernel_Name"] and int(r["Grid_Size"]) <= 256), 0)
+        copies, timed = [], {}
+        for r in rows:
+            if "stream_copy" in r["Kernel_Name"] and int(r["Grid_Size"]) > 256:
+                copies.append(float(r["Counter_Value"]))
+            elif int(r["Dispatch_Id"]) > mark and "ssam2d_kernel" in r["Kernel_Name"]:
+                timed.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return copies, timed
 
-    fetch, write = load(fetch_dir), load(write_dir)
-    copy = next(k for k in fetch if "stream_copy" in k)
+    (fc, fetch), (wc, write) = load(fetch_dir), load(write_dir)
     gib_kib = float(1 << 20)
-    fs = gib_kib / (sum(fetch[copy]) / len(fetch[copy]))
-    ws = gib_kib / (sum(write[copy]) / len(write[copy]))
-    names = {(1 if sh == "box" else 0, r, t): n for n, sh, r, t in SUITE}
+    fs = gib_kib / (sum(fc) / len(fc))
+    ws = gib_kib / (sum(wc) / len(wc))
+    names = {(1 if sh == "box" else 0, r): (n, t) for n, sh, r, t in SUITE}
     alg = 2 * 4 * N * N
     rows = []
     for k in fetch:
@@ -85,10 +138,13 @@ def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
         if not m or k not in write:
             continue
         r, shape, t = (int(v) for v in m.groups())
-        name = names.get((shape, r, t), f"{'box' if shape else 'star'}_r{r}_T{t}")
-        fb = fetch[k][-1] * fs * 1024
-        wb = write[k][-1] * ws * 1024
-        rec = dict(kernel=k.split("(")[0], fetch_bytes=round(fb), write_bytes=round(wb),
+        if (shape, r) not in names:
+            continue
+        name, tsuite = names[(shape, r)]
+        fb = sum(fetch[k]) / len(fetch[k]) * fs * 1024
+        wb = sum(write[k]) / len(write[k]) * ws * 1024
+        rec = dict(kernel=k.split("(")[0], steps_per_launch=t, launches_per_step=tsuite // t,
+                   fetch_bytes=round(fb), write_bytes=round(wb),
                    hbm_bytes_per_launch=round(fb + wb), algorithmic_bytes=alg,
                    traffic_over_algorithmic=round((fb + wb) / alg, 4),
                    scale_fetch=round(fs, 4), scale_write=round(ws, 4))
@@ -101,10 +157,11 @@ def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
           "    python3 bench.py --ingest-pmc <F> <W>", "",
           f"Calibration: the 1 GiB stream copy reads FETCH_SIZE x{fs:.3f} and WRITE_SIZE x{ws:.3f} "
           "of its true bytes (KiB per dispatch); every figure below is scaled by these.", "",
-          "| kernel | fetch MiB | write MiB | HBM / algorithmic |", "|---|---|---|---|"]
+          "| kernel | steps per launch | fetch MiB | write MiB | HBM / algorithmic (per launch) |",
+          "|---|---|---|---|---|"]
     for name, rec in sorted(rows):
-        md.append(f"| {name} | {rec['fetch_bytes'] / 2**20:.1f} | {rec['write_bytes'] / 2**20:.1f} | "
-                  f"{rec['traffic_over_algorithmic']:.3f} |")
+        md.append(f"| {name} | {rec['steps_per_launch']} | {rec['fetch_bytes'] / 2**20:.1f} | "
+                  f"{rec['write_bytes'] / 2**20:.1f} | {rec['traffic_over_algorithmic']:.3f} |")
     (ROOT / "profiles" / "traffic_suite_8192_f32.md").write_text("\n".join(md) + "\n")
     print("\n".join(md))
 
@@ -113,6 +170,9 @@ def main():
     a = parse()
     if a.ingest_pmc:
         ingest_pmc(*a.ingest_pmc)
+        return
+    if a.ingest_trace:
+        ingest_trace(a.ingest_trace, a.steps, a.warmup)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SSAM stencil path runs on a single MI355X here "
@@ -159,6 +219,10 @@ def main():
             tuning[name] = api.tune2d(src, dst, taps(shape, r),
                                       shape=L.STAR if shape == "star" else L.BOX, steps=t, work=work)
     torch.cuda.synchronize()
+    # a 16-byte copy marks the start of the warm-up + timed launches in a kernel
+    # trace (bench.py --ingest-trace keeps the dispatches after it)
+    mark = torch.zeros(8, dtype=torch.float32, device=dev)
+    api.stream_copy(mark[:4], mark[4:])
     for _ in range(a.warmup):
         run_suite(src, dst)
     torch.cuda.synchronize()

@@ -195,9 +195,10 @@ int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t n
 /* ---------------------------------------------------------------- batches
  * Many independent grids of uneven sizes and mixed shapes.  2D items of one
  * storage type whose shape is a 5-point star or a 3x3 / 5x5 / 11x11 box run as
- * ONE persistent launch; 3D items (7-point star or 19-point face+edge, F32/F64)
- * run one persistent launch per sweep for each (storage, steps, shape); any other
- * item gets its own launch.  Persistent launches take tiles from 8 device work
+ * one persistent launch per stencil kind, back to back (or, as the tuner may
+ * choose, ONE persistent launch for all kinds); 3D items (7-point star or 19-point face+edge, F32/F64)
+ * of under 2 GiB run one persistent launch per sweep for each (storage, steps,
+ * shape); any other item gets its own launch.  Persistent launches take tiles from 8 device work
  * queues, one per XCD (neighbouring tiles of a grid share that XCD's L2), and
  * steal from the other queues when their own is empty.  The first ssam_batch_run
  * captures the whole sequence into a hipGraph on a private stream and every run
@@ -227,18 +228,25 @@ int ssam_batch_run(ssam_batch_plan plan, void* stream, uint32_t flags);
 int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles_3d,
                          int64_t* point_updates);
 /* Autotune the plan's tiling on its own buffers (dst and work are overwritten;
- * the stream is synchronised): 2D tile height per shape (32..256 rows) and the 3D
- * tile shape (columns and rows per lane, prefetch depth) and z-segment length,
- * and the queue affinity of each, timed over eager runs; the fastest is kept
- * and the next ssam_batch_run captures a new graph. */
+ * the stream is synchronised): the 2D launch form (one launch per stencil kind,
+ * or one for all) and whether BF16/FP16 5x5 and 11x11 boxes run on the matrix
+ * cores, the 2D tile height per shape (32..256 rows), the 3D tile shape (columns
+ * and rows per lane, prefetch depth) and z-segment length, the persistent grid's
+ * waves per SIMD and the queue affinity of each, and the 3D sweep direction,
+ * timed over eager runs; the fastest is kept and the next ssam_batch_run
+ * captures a new graph. */
 int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
 /* The plan's tiling: 2D tile height per shape (4 entries: 5-point star, 3x3,
- * 5x5, 11x11 box), 3D tile-shape index and z-segment length, and the queue
- * affinity of the 2D and 3D launches (2 entries; 1: waves start at their own
- * XCD's queue, 0: all waves work through the tiles in table order).  Any
+ * 5x5, 11x11 box), 3D tile-shape index and z-segment length, and 7 scheduling
+ * entries: queue affinity of the 2D and of the 3D launches (1: waves start at
+ * their own XCD's queue, 0: all waves work through the tiles in table order),
+ * whether 3D sweeps alternate direction (odd steps walk the table back to front),
+ * the 2D launch form (1: one persistent launch per stencil kind, 0: one for all
+ * kinds), the matrix-core routing of BF16/FP16 boxes (1: on), and the waves per
+ * SIMD of the 2D and of the 3D persistent grids (0: as many as fit).  Any
  * pointer may be NULL. */
 int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* variant_3d,
-                           int32_t* zseg_3d, int32_t* affinity);
+                           int32_t* zseg_3d, int32_t* sched);
 void ssam_batch_plan_destroy(ssam_batch_plan plan);
 
 /* ---------------------------------------------------------------- micro-benchmarks

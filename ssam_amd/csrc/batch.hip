@@ -40,9 +40,12 @@ namespace ssam {
 // ------------------------------------------------------------------ 2D menu
 // Shapes the persistent 2D kernel runs in one launch (mixes of 5-point stars and
 // 3x3 / 5x5 / 11x11 boxes); other shapes fall back to their own launch.
-// k2Box5M: the 11x11 box on the matrix cores (BF16/FP16 storage).
-enum Kind2D : int { k2Star1 = 0, k2Box1 = 1, k2Box2 = 2, k2Box5 = 3, k2Box5M = 4, k2None = -1 };
-constexpr int kNumKind2D = 4;  // tile-height slots (k2Box5M uses k2Box5's)
+// k2Box2M / k2Box5M: the 5x5 / 11x11 box on the matrix cores (BF16/FP16 storage).
+enum Kind2D : int { k2Star1 = 0, k2Box1 = 1, k2Box2 = 2, k2Box5 = 3, k2Box5M = 4, k2Box2M = 5, k2None = -1 };
+constexpr int kNumKind2D = 4;   // tile-height slots (k2Box2M / k2Box5M use k2Box2's / k2Box5's)
+constexpr int kNumLaunch2D = 6;  // launch kinds of the split form (one persistent launch each)
+
+inline int seg_slot(int kind) { return kind == k2Box5M ? k2Box5 : kind == k2Box2M ? k2Box2 : kind; }
 
 inline int kind2d(int shape, int r) {
     if (shape == kStar && r == 1) return k2Star1;
@@ -85,9 +88,11 @@ constexpr int kQueues = 8;  // one per XCD
 // First tile index of each queue (queue q holds tiles [beg[q], beg[q+1])).
 // affinity 1: a wave starts at its own XCD's queue; 0: every wave starts at
 // queue 0, so the whole device works through the tiles in table order.
+// alternate (3D): odd steps walk the table back to front.
 struct QueueSet {
     int beg[kQueues + 1];
     int affinity;
+    int alternate;
 };
 
 // Physical XCD of this wave (hardware register XCC_ID, bits 3:0).
@@ -95,34 +100,101 @@ __device__ __forceinline__ int xcd_id() {
     return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kQueues - 1);
 }
 
-__device__ __forceinline__ int next_tile(int* counter) {
-    int t = 0;
+// Each queue's counter sits on a 256-byte line of its own: atomics on one
+// address serialise where they execute, atomics on different lines do not.
+constexpr int kCounterStride = 64;                      // ints
+constexpr int kCounterSet = kQueues * kCounterStride;   // ints per launch
+
+// Lane 0 claims the next tile of a queue (one atomic); other lanes return len.
+__device__ __forceinline__ int claim(int* counter, int len) {
+    int t = len;
     if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(counter, 1);
-    return __builtin_amdgcn_readfirstlane(t);
+    return t;
+}
+
+// The same for a queue that is likely drained already (another XCD's): one
+// relaxed load first, so finding it empty costs no atomic.
+__device__ __forceinline__ int claim_checked(int* counter, int len) {
+    int t = len;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len)
+            t = atomicAdd(counter, 1);
+    }
+    return t;
 }
 
 // Run body(tile) for tiles of this XCD's queue, then of the others in turn,
-// until every queue is empty.  counters: one per queue, zero at launch.
-template <typename F>
-__device__ __forceinline__ void drain_queues(const QueueSet& qs, int* __restrict__ counters, F&& body) {
+// until every queue is empty.  counters: kCounterSet ints, zero at launch.
+// reverse: walk queues and tiles back to front (the 3D sweeps alternate, so a
+// step starts on the grids the previous step wrote last, still in the last-level
+// cache).  AHEAD: the claim of a wave's next tile is issued before its current
+// tile runs, so the atomic's round trip overlaps the tile (for short tiles; with
+// few long tiles per wave, a tile held in reserve by a busy wave lengthens the
+// tail).
+template <bool AHEAD, typename F>
+__device__ __forceinline__ void drain_queues(const QueueSet& qs, int* __restrict__ counters, bool reverse,
+                                             F&& body) {
     const int x = qs.affinity ? xcd_id() : 0;
-    for (int k = 0; k < kQueues; ++k) {
-        const int q = (x + k) & (kQueues - 1);
-        const int beg = qs.beg[q], end = qs.beg[q + 1];
-        if (beg >= end) continue;
-        for (;;) {
-            const int t = beg + next_tile(counters + q);
-            if (t >= end) break;
-            body(t);
+    // one flat loop (a nested queue loop around the tile body costs the tile
+    // registers): k counts the queues already drained
+    int k = 0;
+    int q = reverse ? kQueues - 1 - x : x;
+    int beg = qs.beg[q], len = qs.beg[q + 1] - beg;
+    int n = __builtin_amdgcn_readfirstlane(claim(counters + q * kCounterStride, len));
+    for (;;) {
+        if (n >= len) {
+            if (++k == kQueues) break;
+            const int qq = (x + k) & (kQueues - 1);
+            q = reverse ? kQueues - 1 - qq : qq;
+            beg = qs.beg[q];
+            len = qs.beg[q + 1] - beg;
+            n = __builtin_amdgcn_readfirstlane(claim_checked(counters + q * kCounterStride, len));
+            continue;
+        }
+        if constexpr (AHEAD) {
+            const int next = claim(counters + q * kCounterStride, len);
+            body(reverse ? beg + len - 1 - n : beg + n);
+            n = __builtin_amdgcn_readfirstlane(next);
+        } else {
+            body(reverse ? beg + len - 1 - n : beg + n);
+            n = __builtin_amdgcn_readfirstlane(claim(counters + q * kCounterStride, len));
         }
     }
 }
 
-template <typename S>
-__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(2)))
+template <typename S> constexpr bool kHalf2D = std::is_same<S, bf16_t>::value || std::is_same<S, f16_t>::value;
+
+// One tile of one launch kind.
+template <typename S, int KIND>
+__device__ __forceinline__ void run_tile2d(const S* src, S* dst, const BatchItem2D& b,
+                                           const typename Storage<S>::acc* w, int strip, int seg) {
+    if constexpr (KIND == k2Star1) Inst2D<S, 1, kStar>::K::run_tile(src, dst, b.p, w, strip, seg);
+    else if constexpr (KIND == k2Box1) Inst2D<S, 1, kBox>::K::run_tile(src, dst, b.p, w, strip, seg);
+    else if constexpr (KIND == k2Box2) Inst2D<S, 2, kBox>::K::run_tile(src, dst, b.p, w, strip, seg);
+    else if constexpr (KIND == k2Box5) Inst2D<S, 5, kBox>::K::run_tile(src, dst, b.p, w, strip, seg);
+    else if constexpr (kHalf2D<S> && KIND == k2Box2M)
+        Mfma2D<S, 2, std::is_same<S, f16_t>::value, 2, 1>::run_tile(src, dst, b.pm, w, strip, seg);
+    else if constexpr (kHalf2D<S> && KIND == k2Box5M)
+        Mfma2D<S, 5, std::is_same<S, f16_t>::value, 2, 1>::run_tile(src, dst, b.pm, w, strip, seg);
+}
+
+// Waves per SIMD the register allocation leaves room for: the matrix-core tiles
+// as their single-grid launch (3 with one set of taps, 2 with hi + lo taps), the
+// rest 2 (the fused form holds the largest tile body of all kinds).
+template <typename S, int KIND> struct Waves2D {
+    static constexpr int value =
+        (KIND == k2Box2M || KIND == k2Box5M) ? (std::is_same<S, f16_t>::value ? 2 : 3) : 2;
+};
+
+// KIND >= 0: every tile of the launch has that kind (the split form, one launch
+// per kind); KIND < 0: the fused form, one launch for all kinds, dispatching per
+// item.
+template <typename S, int KIND>
+__global__ __launch_bounds__(kWave * kBatchWavesPerBlock)
+__attribute__((amdgpu_waves_per_eu(Waves2D<S, KIND>::value)))
 void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
                     int* __restrict__ counters, const typename Storage<S>::acc* __restrict__ taps) {
-    drain_queues(qs, counters, [&](int t) {
+    drain_queues<true>(qs, counters, false, [&](int t) {
         const int4 e = tiles[t];
         const int it = __builtin_amdgcn_readfirstlane(e.x);
         const int strip = __builtin_amdgcn_readfirstlane(e.y);
@@ -131,15 +203,17 @@ void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restric
         const S* src = static_cast<const S*>(b.src);
         S* dst = static_cast<S*>(b.dst);
         const typename Storage<S>::acc* w = taps + b.taps_off;
-        switch (b.kind) {
-            case k2Star1: Inst2D<S, 1, kStar>::K::run_tile(src, dst, b.p, w, strip, seg); break;
-            case k2Box1: Inst2D<S, 1, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
-            case k2Box2: Inst2D<S, 2, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
-            case k2Box5M:
-                if constexpr (std::is_same<S, bf16_t>::value || std::is_same<S, f16_t>::value)
-                    Mfma2D<S, 5, std::is_same<S, f16_t>::value, 2, 1>::run_tile(src, dst, b.pm, w, strip, seg);
-                break;
-            default: Inst2D<S, 5, kBox>::K::run_tile(src, dst, b.p, w, strip, seg); break;
+        if constexpr (KIND >= 0) {
+            run_tile2d<S, KIND>(src, dst, b, w, strip, seg);
+        } else {
+            switch (b.kind) {
+                case k2Star1: run_tile2d<S, k2Star1>(src, dst, b, w, strip, seg); break;
+                case k2Box1: run_tile2d<S, k2Box1>(src, dst, b, w, strip, seg); break;
+                case k2Box2: run_tile2d<S, k2Box2>(src, dst, b, w, strip, seg); break;
+                case k2Box2M: run_tile2d<S, k2Box2M>(src, dst, b, w, strip, seg); break;
+                case k2Box5M: run_tile2d<S, k2Box5M>(src, dst, b, w, strip, seg); break;
+                default: run_tile2d<S, k2Box5>(src, dst, b, w, strip, seg); break;
+            }
         }
     });
 }
@@ -147,9 +221,11 @@ void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restric
 // Batched 3D tile shapes: (columns per lane, rows per lane, planes prefetched).
 // Narrow grids favour few columns per lane (62*V-column strips waste little of a
 // 64..256-column row); more rows per lane cut the y-halo re-reads.
-struct Var3D { int v, ny, pf; };
-constexpr Var3D kVar3D[] = {{1, 8, 2}, {1, 4, 3}, {2, 4, 2}, {2, 8, 1},
-                            {2, 8, 2}, {4, 4, 1}, {4, 4, 2}, {4, 8, 1}};
+// wpe: waves per SIMD the register allocation must leave room for (bounds the
+// compiler's load hoisting to what each shape needs).
+struct Var3D { int v, ny, pf, wpe; };
+constexpr Var3D kVar3D[] = {{1, 8, 2, 4}, {2, 16, 1, 2}, {2, 4, 2, 4}, {2, 8, 1, 3},
+                            {2, 8, 2, 2}, {2, 12, 2, 2}, {1, 16, 2, 2}, {4, 8, 1, 2}};
 constexpr int kNumVar3D = 8;
 constexpr int kDefaultVar3D = 4;
 constexpr int kDefaultZseg3D = 32;
@@ -159,15 +235,16 @@ template <typename S, int VAR> struct Var3DT {
     // FP64 runs one column per lane (a plane row of NY+2 doubles per lane)
     static constexpr int V = sizeof(S) == 8 ? 1 : kVar3D[VAR].v;
     static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
+    static constexpr int WPE = sizeof(S) == 8 ? (NY >= 16 ? 1 : 2) : kVar3D[VAR].wpe;
 };
 
 // tile.x item, .y strip, .z y-block, .w first plane (low 20 bits) | planes << 20
 template <typename S, int VAR, int SHAPE>
-__global__ __launch_bounds__(kWave * kBatchWavesPerBlock)
+__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(Var3DT<S, VAR>::WPE)))
 void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
                     int* __restrict__ counters, const S* __restrict__ taps, int step) {
     using T = Var3DT<S, VAR>;
-    drain_queues(qs, counters, [&](int t) {
+    drain_queues<false>(qs, counters, qs.alternate && (step & 1), [&](int t) {
         const int4 e = tiles[t];
         const int it = __builtin_amdgcn_readfirstlane(e.x);
         const int strip = __builtin_amdgcn_readfirstlane(e.y);
@@ -204,7 +281,14 @@ struct ssam_batch_plan_s {
         void* d_tiles = nullptr;
         void* d_taps = nullptr;
         int ntiles = 0;
-        QueueSet qs{};
+        QueueSet qs{};    // 3D
+        struct Sub {      // 2D: one persistent launch per kind (split) or one for all (fused)
+            int kind = -1;
+            int tile_off = 0, ntiles = 0;
+            QueueSet qs{};
+            int blocks = 1;
+        };
+        std::vector<Sub> subs;
         int steps = 1;    // 3D
         int kind = 0;     // 3D stencil: 0 7-point star, 1 19-point face+edge
         int blocks = 1;   // persistent grid: resident workgroups of its kernel
@@ -216,6 +300,10 @@ struct ssam_batch_plan_s {
     int seg2d[kNumKind2D] = {kDefaultSeg2D, kDefaultSeg2D, kDefaultSeg2D, kDefaultSeg2D};
     int var3d = kDefaultVar3D, zseg3d = kDefaultZseg3D;
     int aff2d = 1, aff3d = 0;  // queue affinity (QueueSet::affinity)
+    int alt3d = 1;             // 3D sweeps alternate direction (QueueSet::alternate)
+    int split2d = 1;           // 2D: one launch per kind (1) or one for all kinds (0)
+    int mfma2d = 1;            // BF16/FP16 5x5 and 11x11 boxes on the matrix cores
+    int wps2d = 0, wps3d = 1;  // persistent grid: waves per SIMD (0: as many as fit)
     int ncounters = 0;
     void* d_counters = nullptr;
     int64_t tiles2d = 0, tiles3d = 0, point_updates = 0;
@@ -243,8 +331,9 @@ int env_int(const char* name, int dflt) {
     return v && *v ? std::atoi(v) : dflt;
 }
 
-// Workgroups of `fn` the device holds at once (the persistent grid).
-int resident_blocks(const void* fn) {
+// The persistent grid of `fn`: the workgroups the device holds at once, at
+// most `wps` per CU (one wave per SIMD each; 0: no cap).
+int resident_blocks(const void* fn, int wps) {
     const int forced = env_int("SSAM_BATCH_BLOCKS", 0);
     if (forced > 0) return forced;
     int dev = 0, cus = 0, per = 0;
@@ -255,6 +344,7 @@ int resident_blocks(const void* fn) {
             hipSuccess ||
         per < 1)
         per = 1;
+    if (wps > 0 && wps < per) per = wps;
     return cus * per;
 }
 
@@ -277,7 +367,17 @@ template <typename S> const void* batch3d_fn(int variant, int kind) {
     return reinterpret_cast<const void*>(kind ? batch3d_fn_shape<S, k3FaceEdge>(variant)
                                               : batch3d_fn_shape<S, k3Star>(variant));
 }
-template <typename S> const void* batch2d_fn() { return reinterpret_cast<const void*>(batch2d_kernel<S>); }
+template <typename S> const void* batch2d_fn(int kind) {
+    switch (kind) {
+        case k2Star1: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Star1>);
+        case k2Box1: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box1>);
+        case k2Box2: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box2>);
+        case k2Box5: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box5>);
+        case k2Box2M: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box2M>);
+        case k2Box5M: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box5M>);
+        default: return reinterpret_cast<const void*>(batch2d_kernel<S, -1>);
+    }
+}
 
 // One item's tiles in memory order with a per-tile cost estimate.
 struct ItemTiles {
@@ -373,9 +473,9 @@ void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p,
     }
 }
 
-// Tiles of the matrix-core 11x11 box: 32-column strips, rows in whole ring
-// periods (64).
-void tiles_for_mfma(const ssam_batch_item& it, int idx, int seg_rows, ParamsM* p, ItemTiles* out) {
+// Tiles of the matrix-core boxes: 32-column strips, rows in whole ring periods
+// (64).
+void tiles_for_mfma(const ssam_batch_item& it, int idx, int R, int seg_rows, ParamsM* p, ItemTiles* out) {
     constexpr int OW = 32;
     p->width = static_cast<int>(it.nx);
     p->height = static_cast<int>(it.ny);
@@ -389,8 +489,10 @@ void tiles_for_mfma(const ssam_batch_item& it, int idx, int seg_rows, ParamsM* p
         const int rows = std::min(p->seg_rows, p->height - sg * p->seg_rows);
         for (int st = 0; st < p->nstrips; ++st) {
             out->tiles.push_back(int4{idx, st, sg, 0});
-            // a quarter of the VALU kernel's cost per point (measured ratio)
-            out->cost.push_back(static_cast<double>(rows + 10) * (121.0 + 8.0) * 0.25 * OW / 248.0);
+            // a quarter of the VALU kernel's cost per point (measured ratio on
+            // the 11x11 box)
+            const double taps = (2.0 * R + 1) * (2.0 * R + 1);
+            out->cost.push_back(static_cast<double>(rows + 2 * R) * (taps + 8.0) * 0.25 * OW / 248.0);
         }
     }
 }
@@ -402,7 +504,9 @@ void free_group(Plan::Group& g) {
     g.d_items = g.d_tiles = g.d_taps = nullptr;
 }
 
-// (Re)build the device tables of a 2D group with the plan's tile heights.
+// (Re)build the device tables of a 2D group with the plan's tile heights: one
+// queue set over all tiles (fused) or one per launch kind (split), the kinds'
+// tiles one after the other in one table.
 template <typename S>
 int build_group2d(Plan* P, Plan::Group& g) {
     using A = typename Storage<S>::acc;
@@ -426,11 +530,12 @@ int build_group2d(Plan* P, Plan::Group& g) {
             }
         const int idx = static_cast<int>(k);
         const int seg = P->seg2d[b.kind];
-        // 11x11 boxes in BF16/FP16 go to the matrix cores when rows are 16-byte aligned
-        if (b.kind == k2Box5 && (std::is_same<S, bf16_t>::value || std::is_same<S, f16_t>::value) &&
-            it.pitch % 8 == 0 && ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 16) == 0) {
-            b.kind = k2Box5M;
-            tiles_for_mfma(it, idx, seg, &b.pm, &per[k]);
+        // 5x5 and 11x11 boxes in BF16/FP16 go to the matrix cores when rows are
+        // 16-byte aligned
+        if (P->mfma2d && (b.kind == k2Box2 || b.kind == k2Box5) && kHalf2D<S> && it.pitch % 8 == 0 &&
+            ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 16) == 0) {
+            b.kind = b.kind == k2Box2 ? k2Box2M : k2Box5M;
+            tiles_for_mfma(it, idx, R, seg, &b.pm, &per[k]);
             continue;
         }
         switch (b.kind) {
@@ -441,11 +546,31 @@ int build_group2d(Plan* P, Plan::Group& g) {
         }
     }
     std::vector<int4> tiles;
-    pack_queues(per, &tiles, &g.qs);
-    g.qs.affinity = P->aff2d;
+    g.subs.clear();
+    const int nkinds = P->split2d ? kNumLaunch2D : 1;
+    for (int kd = 0; kd < nkinds; ++kd) {
+        std::vector<ItemTiles> part(per.size());
+        bool any = false;
+        for (size_t k = 0; k < per.size(); ++k)
+            if (!P->split2d || h[k].kind == kd) {
+                part[k] = per[k];
+                any = any || !per[k].tiles.empty();
+            }
+        if (!any) continue;
+        Plan::Group::Sub sb;
+        sb.kind = P->split2d ? kd : -1;
+        std::vector<int4> t;
+        pack_queues(part, &t, &sb.qs);
+        sb.qs.affinity = P->aff2d;
+        sb.qs.alternate = 0;
+        sb.tile_off = static_cast<int>(tiles.size());
+        sb.ntiles = static_cast<int>(t.size());
+        sb.blocks = resident_blocks(batch2d_fn<S>(sb.kind), P->wps2d);
+        tiles.insert(tiles.end(), t.begin(), t.end());
+        g.subs.push_back(sb);
+    }
     free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
-    g.blocks = resident_blocks(batch2d_fn<S>());
     int rc = upload(&g.d_items, h);
     if (rc == SSAM_OK) rc = upload(&g.d_tiles, tiles);
     if (rc == SSAM_OK) rc = upload(&g.d_taps, taps);
@@ -498,9 +623,10 @@ int build_group3d(Plan* P, Plan::Group& g) {
     std::vector<int4> tiles;
     pack_queues(per, &tiles, &g.qs);
     g.qs.affinity = P->aff3d;
+    g.qs.alternate = P->alt3d;
     free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
-    g.blocks = resident_blocks(batch3d_fn<S>(variant, g.kind));
+    g.blocks = resident_blocks(batch3d_fn<S>(variant, g.kind), P->wps3d);
     int rc = upload(&g.d_items, h);
     if (rc == SSAM_OK) rc = upload(&g.d_tiles, tiles);
     if (rc == SSAM_OK) rc = upload(&g.d_taps, taps);
@@ -554,35 +680,35 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
                        static_cast<int*>(P->d_counters), P->ncounters);
     if (hipGetLastError() != hipSuccess) return SSAM_E_HIP;
     int* counters = static_cast<int*>(P->d_counters);
-    int ctr = 0;  // counter sets (kQueues counters each): 2D groups, then 3D (step, group)
+    int ctr = 0;  // counter sets (kCounterSet ints each): 2D groups, then 3D (step, group)
     for (auto& g : P->groups) {
         if (g.ndim != 2) continue;
-        int* cnt = counters + kQueues * ctr++;
+        // kNumLaunch2D counter sets per 2D group, whatever its form
+        int* cnt = counters + kCounterSet * ctr;
+        ctr += kNumLaunch2D;
         if (!(parts & kRun2D)) continue;
-        const int blocks = std::max(1, std::min(g.blocks, (g.ntiles + kBatchWavesPerBlock - 1) /
-                                                              kBatchWavesPerBlock));
-        const dim3 grid(blocks), block(kWave * kBatchWavesPerBlock);
-        const auto* items = static_cast<const BatchItem2D*>(g.d_items);
-        const auto* tiles = static_cast<const int4*>(g.d_tiles);
-        switch (g.storage) {
-            case SSAM_F32:
-                hipLaunchKernelGGL(batch2d_kernel<float>, grid, block, 0, st, items, tiles, g.qs, cnt,
-                                   static_cast<const float*>(g.d_taps));
-                break;
-            case SSAM_F64:
-                hipLaunchKernelGGL(batch2d_kernel<double>, grid, block, 0, st, items, tiles, g.qs, cnt,
-                                   static_cast<const double*>(g.d_taps));
-                break;
-            case SSAM_BF16:
-                hipLaunchKernelGGL(batch2d_kernel<bf16_t>, grid, block, 0, st, items, tiles, g.qs, cnt,
-                                   static_cast<const float*>(g.d_taps));
-                break;
-            default:
-                hipLaunchKernelGGL(batch2d_kernel<f16_t>, grid, block, 0, st, items, tiles, g.qs, cnt,
-                                   static_cast<const float*>(g.d_taps));
-                break;
+        for (size_t k = 0; k < g.subs.size(); ++k) {
+            const Plan::Group::Sub& sb = g.subs[k];
+            const int blocks = std::max(1, std::min(sb.blocks, (sb.ntiles + kBatchWavesPerBlock - 1) /
+                                                                   kBatchWavesPerBlock));
+            const void* items = g.d_items;
+            const void* tiles = static_cast<const int4*>(g.d_tiles) + sb.tile_off;
+            QueueSet qs = sb.qs;
+            int* c = cnt + kCounterSet * k;
+            const void* taps = g.d_taps;
+            void* args[] = {&items, &tiles, &qs, &c, &taps};
+            const void* fn = nullptr;
+            switch (g.storage) {
+                case SSAM_F32: fn = batch2d_fn<float>(sb.kind); break;
+                case SSAM_F64: fn = batch2d_fn<double>(sb.kind); break;
+                case SSAM_BF16: fn = batch2d_fn<bf16_t>(sb.kind); break;
+                default: fn = batch2d_fn<f16_t>(sb.kind); break;
+            }
+            if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
+                    hipSuccess ||
+                hipGetLastError() != hipSuccess)
+                return SSAM_E_HIP;
         }
-        if (hipGetLastError() != hipSuccess) return SSAM_E_HIP;
     }
     if (parts & kRunFallback) {
         for (int i : P->fallback) {
@@ -616,7 +742,7 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
     for (int s = 0; s < maxsteps; ++s) {
         for (auto& g : P->groups) {
             if (g.ndim != 3) continue;
-            int* cnt = counters + kQueues * ctr++;
+            int* cnt = counters + kCounterSet * ctr++;
             if (!(parts & kRun3D) || s >= g.steps) continue;
             const int blocks = std::max(1, std::min(g.blocks, (g.ntiles + kBatchWavesPerBlock - 1) /
                                                                   kBatchWavesPerBlock));
@@ -693,7 +819,8 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
             const char* a = static_cast<const char*>(it.src);
             const char* b = static_cast<const char*>(it.dst);
             if (a < b + bytes && b < a + bytes) return SSAM_E_INVALID;
-            if (it.radius == 1 && (it.shape == SSAM_SHAPE_STAR || it.shape == SSAM_SHAPE_FACE_EDGE)) {
+            if (it.radius == 1 && (it.shape == SSAM_SHAPE_STAR || it.shape == SSAM_SHAPE_FACE_EDGE) &&
+                grid3d_fits_buffer(it.nx, it.ny, it.nz, es)) {
                 // tile fields: first plane in 20 bits, plane count in 11
                 if (it.nz >= (1 << 20) || it.nx > (1 << 30) || it.ny > (1 << 30))
                     return SSAM_E_UNSUPPORTED;
@@ -735,7 +862,12 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
     const int seg = env_int("SSAM_BATCH2D_SEG", kDefaultSeg2D);
     for (int k = 0; k < kNumKind2D; ++k) P->seg2d[k] = seg >= 8 ? seg : kDefaultSeg2D;
     P->aff2d = env_int("SSAM_BATCH2D_AFFINITY", 1) != 0;
+    P->split2d = env_int("SSAM_BATCH2D_SPLIT", 1) != 0;
+    P->mfma2d = env_int("SSAM_BATCH2D_MFMA", 1) != 0;
+    P->wps2d = std::max(0, env_int("SSAM_BATCH2D_WPS", 0));
+    P->wps3d = std::max(0, env_int("SSAM_BATCH3D_WPS", 1));
     P->aff3d = env_int("SSAM_BATCH3D_AFFINITY", 0) != 0;
+    P->alt3d = env_int("SSAM_BATCH3D_ALTERNATE", 1) != 0;
     for (int s = 0; s < 4; ++s) {
         if (g2[s].empty()) continue;
         Plan::Group g;
@@ -756,7 +888,8 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
     }
     int rc = build_all(P);
     if (rc == SSAM_OK) {
-        // kQueues counters per 2D launch and per (3D group, step) slot of enqueue()
+        // one counter set per 2D launch kind of each 2D group and per (3D group,
+        // step) slot of enqueue()
         int n2 = 0, n3 = 0, maxsteps = 0;
         for (auto& g : P->groups) {
             if (g.ndim == 2) ++n2;
@@ -765,7 +898,7 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
                 maxsteps = std::max(maxsteps, g.steps);
             }
         }
-        P->ncounters = std::max(1, kQueues * (n2 + n3 * maxsteps));
+        P->ncounters = std::max(1, kCounterSet * (n2 * kNumLaunch2D + n3 * maxsteps));
         if (hipMalloc(&P->d_counters, P->ncounters * sizeof(int)) != hipSuccess) rc = SSAM_E_HIP;
     }
     if (rc != SSAM_OK) {
@@ -821,76 +954,58 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
             }
         }
     }
-    float ms = 0.f;
     int rc = SSAM_OK;
-    // 2D: tile height per shape, one shape at a time (the others held), then
-    // queue affinity
-    const int segs[] = {32, 64, 128, 256};
-    for (int k = 0; k < kNumKind2D && has2 && rc == SSAM_OK; ++k) {
-        if (!kinds[k]) continue;
-        int best = P->seg2d[k];
+    // Time every combination of candidate values of the given fields (parts of
+    // the run only) and keep the fastest.
+    auto search = [&](std::vector<int*> fields, std::vector<std::vector<int>> cands, unsigned parts) {
+        std::vector<int> best(fields.size()), idx(fields.size(), 0);
+        for (size_t f = 0; f < fields.size(); ++f) best[f] = *fields[f];
         float best_ms = 1e30f;
-        for (int s : segs) {
-            P->seg2d[k] = s;
+        while (rc == SSAM_OK) {
+            for (size_t f = 0; f < fields.size(); ++f) *fields[f] = cands[f][idx[f]];
+            float ms = 0.f;
             rc = build_all(P);
-            if (rc == SSAM_OK) rc = time_parts(P, st, kRun2D, reps, &ms);
-            if (rc != SSAM_OK) break;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = s;
-            }
-        }
-        P->seg2d[k] = best;
-    }
-    if (has2 && rc == SSAM_OK) {
-        int best = P->aff2d;
-        float best_ms = 1e30f;
-        for (int a = 0; a < 2 && rc == SSAM_OK; ++a) {
-            P->aff2d = a;
-            rc = build_all(P);
-            if (rc == SSAM_OK) rc = time_parts(P, st, kRun2D, reps, &ms);
+            if (rc == SSAM_OK) rc = time_parts(P, st, parts, reps, &ms);
             if (rc == SSAM_OK && ms < best_ms) {
                 best_ms = ms;
-                best = a;
+                for (size_t f = 0; f < fields.size(); ++f) best[f] = *fields[f];
             }
+            size_t f = 0;  // next combination (odometer)
+            while (f < fields.size() && ++idx[f] == static_cast<int>(cands[f].size())) idx[f++] = 0;
+            if (f == fields.size()) break;
         }
-        P->aff2d = best;
+        for (size_t f = 0; f < fields.size(); ++f) *fields[f] = best[f];
+    };
+    if (has2) {
+        // launch form x matrix-core routing x persistent-grid occupancy, then the
+        // tile height per shape (one shape at a time, the others held), then
+        // queue affinity
+        search({&P->split2d, &P->mfma2d, &P->wps2d}, {{0, 1}, {0, 1}, {1, 2, 4, 0}}, kRun2D);
+        for (int k = 0; k < kNumKind2D; ++k)
+            if (kinds[k]) search({&P->seg2d[k]}, {{32, 64, 128, 256}}, kRun2D);
+        search({&P->aff2d}, {{0, 1}}, kRun2D);
     }
-    // 3D: tile shape x z-segment length x queue affinity
-    if (has3 && rc == SSAM_OK) {
-        const int vars[] = {2, 3, 4};
-        const int zsegs[] = {16, 32};
-        int bv = P->var3d, bz = P->zseg3d, ba = P->aff3d;
-        float best_ms = 1e30f;
-        for (int v : vars)
-            for (int z : zsegs)
-                for (int a = 0; a < 2 && rc == SSAM_OK; ++a) {
-                    P->var3d = v;
-                    P->zseg3d = z;
-                    P->aff3d = a;
-                    rc = build_all(P);
-                    if (rc == SSAM_OK) rc = time_parts(P, st, kRun3D, reps, &ms);
-                    if (rc == SSAM_OK && ms < best_ms) {
-                        best_ms = ms;
-                        bv = v;
-                        bz = z;
-                        ba = a;
-                    }
-                }
-        P->var3d = bv;
-        P->zseg3d = bz;
-        P->aff3d = ba;
+    if (has3) {
+        // tile shape x z-segment length x occupancy, then queue affinity and
+        // step direction
+        search({&P->var3d, &P->zseg3d, &P->wps3d}, {{0, 1, 2, 3, 4, 5, 6, 7}, {16, 32, 64}, {1, 2, 4}}, kRun3D);
+        search({&P->aff3d, &P->alt3d}, {{0, 1}, {0, 1}}, kRun3D);
     }
     const int rc2 = build_all(P);
     return rc != SSAM_OK ? rc : rc2;
 }
 
 int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* variant_3d,
-                           int32_t* zseg_3d, int32_t* affinity) {
+                           int32_t* zseg_3d, int32_t* sched) {
     if (!plan) return SSAM_E_INVALID;
-    if (affinity) {
-        affinity[0] = plan->aff2d;
-        affinity[1] = plan->aff3d;
+    if (sched) {
+        sched[0] = plan->aff2d;
+        sched[1] = plan->aff3d;
+        sched[2] = plan->alt3d;
+        sched[3] = plan->split2d;
+        sched[4] = plan->mfma2d;
+        sched[5] = plan->wps2d;
+        sched[6] = plan->wps3d;
     }
     if (seg_rows_2d)
         for (int k = 0; k < kNumKind2D; ++k) seg_rows_2d[k] = plan->seg2d[k];

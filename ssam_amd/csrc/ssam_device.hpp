@@ -107,6 +107,49 @@ template <typename T> __device__ __forceinline__ void gst(void* p, T v) {
     *(__attribute__((address_space(1))) R*)p = r;
 }
 
+// ---------------------------------------------------------------- raw buffers
+// Buffer resource over [base, base + bytes) (bytes < 2^31).  An access whose
+// per-lane byte offset is at or past `bytes` reads 0 and drops its store (the
+// hardware range check of raw buffers, offset = VGPR offset + immediate), so an
+// offset built from an out-of-grid coordinate, which wraps to >= 2^31, never
+// touches memory outside the buffer.  The SGPR offset operand is always 0.
+using BufRsrc = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t kBufOOB = 0x80000000u;  // an offset no buffer here reaches
+
+__device__ __forceinline__ BufRsrc buf_rsrc(const void* base, uint32_t bytes) {
+    // dword 3: 32-bit data format, the value for gfx9-family raw access
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0),
+                                             static_cast<int>(bytes), 0x00020000);
+}
+
+template <int N> struct BufWords;
+template <> struct BufWords<1> { using type = uint32_t; };
+template <> struct BufWords<2> { typedef uint32_t type __attribute__((ext_vector_type(2))); };
+template <> struct BufWords<4> { typedef uint32_t type __attribute__((ext_vector_type(4))); };
+
+// N consecutive 4-byte elements (or N/2 doubles) at byte offset `off`
+template <typename T, int N>
+__device__ __forceinline__ void buf_load(BufRsrc r, uint32_t off, T (&out)[N]) {
+    constexpr int W = N * static_cast<int>(sizeof(T)) / 4;
+    static_assert(W == 1 || W == 2 || W == 4, "1, 2 or 4 dwords");
+    typename BufWords<W>::type v;
+    if constexpr (W == 1) v = __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(off), 0, 0);
+    else if constexpr (W == 2) v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(off), 0, 0);
+    else v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(off), 0, 0);
+    __builtin_memcpy(&out[0], &v, sizeof(v));
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void buf_store(BufRsrc r, uint32_t off, const T (&in)[N]) {
+    constexpr int W = N * static_cast<int>(sizeof(T)) / 4;
+    static_assert(W == 1 || W == 2 || W == 4, "1, 2 or 4 dwords");
+    typename BufWords<W>::type v;
+    __builtin_memcpy(&v, &in[0], sizeof(v));
+    if constexpr (W == 1) __builtin_amdgcn_raw_buffer_store_b32(v, r, static_cast<int>(off), 0, 0);
+    else if constexpr (W == 2) __builtin_amdgcn_raw_buffer_store_b64(v, r, static_cast<int>(off), 0, 0);
+    else __builtin_amdgcn_raw_buffer_store_b128(v, r, static_cast<int>(off), 0, 0);
+}
+
 // ---------------------------------------------------------------- strip I/O
 // One lane's V consecutive elements of a row.  `full` = the whole strip lies
 // inside [0, width) and is aligned for a single vector access.

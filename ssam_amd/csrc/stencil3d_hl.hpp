@@ -6,8 +6,15 @@
 // change of layout that suits narrow grids: the strip's outer lanes are halo
 // lanes.  Lane l of strip s holds columns x0 = s*OW + (l-1)*V .. x0+V-1 with
 // OW = 62*V; lanes 0 and 63 only feed their neighbours and store nothing.
-// Every load address is clamped into the grid, so the hot loop has no edge
-// branch and no extra halo load per row; stores are masked per lane.
+//
+// Addressing: the grid is one raw buffer (< 2 GiB) and every access is a 32-bit
+// byte offset, the row part wave-uniform, the column part per lane.  A row or
+// column outside the grid gives an offset that either wraps past the buffer end
+// (the load reads 0) or lands on another row of the grid; either way the value
+// only reaches outputs of the boundary layer, which are written as 0.  So the
+// hot loop has no clamp, no edge branch and no 64-bit address arithmetic; a lane
+// that stores nothing gets an out-of-range store offset, and only the rows of the
+// last y-block below the grid skip their store (a wave-uniform branch).
 //
 // Output rule (zero Dirichlet, as stencil3d.hpp): interior points get the
 // stencil, the outer layer of the grid gets 0.
@@ -21,47 +28,54 @@ namespace ssam {
 
 struct Grid3D {
     int nx, ny, nz;
-    long long sy, sz;  // element strides of y and z
+    long long sy, sz;  // element strides of y and z (dense: nx, nx*ny)
 };
+
+// Whether a grid fits the 32-bit byte offsets of the batched tile.
+inline bool grid3d_fits_buffer(long long nx, long long ny, long long nz, int elem_bytes) {
+    return nx * ny * nz * elem_bytes < (1LL << 31) - 4096;
+}
 
 template <typename S, int SHAPE, int V, int NY, int PF>
 struct Stencil3DHL {
     using A = typename Storage<S>::acc;
+    static_assert(sizeof(S) == sizeof(A), "F32 / F64 storage");
     static constexpr int R = 1;
     static constexpr int NT = ntaps3(SHAPE, R);
     static constexpr int NROW = NY + 2;     // rows cached per plane
     static constexpr int NP = 3 + PF;       // planes in the ring
     static constexpr int OW = (kWave - 2) * V;
+    static constexpr uint32_t ES = sizeof(S);
 
     struct Plane { A v[NROW][V]; };
 
     struct Ctx {
-        const S* src;
-        S* dst;
-        Grid3D g;
-        int x0, y0;
-        uint32_t xoff;        // byte offset of the lane's (clamped) first column
-        uint32_t xe[V];       // per-element clamped byte offsets (VEC = false)
-        bool store_lane;
+        BufRsrc src, dst;
+        int ny, nz;
+        uint32_t sy, sz;      // byte strides
+        int y0;
+        uint32_t lo;          // byte offset of the lane's first column in its row
+        uint32_t so[V];       // store offset of each column (kBufOOB: not stored)
+        bool xin[V];          // interior column
     };
 
-    // VEC: rows are V-aligned (nx % V == 0, aligned base), one vector access per
-    // row; otherwise V dword accesses.  Both are branch-free.
+    // VEC: rows are V-aligned (nx % V == 0), one V-wide access per row;
+    // otherwise V single-element accesses.
     template <bool VEC>
     __device__ static __forceinline__ void load_plane(const Ctx& c, int z, Plane& P) {
-        const int zz = clampi(z, 0, c.g.nz - 1);
-        const char* base = reinterpret_cast<const char*>(c.src) +
-                           static_cast<long long>(zz) * c.g.sz * static_cast<long long>(sizeof(S));
+        const uint32_t pz = static_cast<uint32_t>(z) * c.sz + static_cast<uint32_t>(c.y0 - 1) * c.sy;
 #pragma unroll
         for (int i = 0; i < NROW; ++i) {
-            const int y = clampi(c.y0 - 1 + i, 0, c.g.ny - 1);
-            const char* row = base + static_cast<long long>(y) * c.g.sy * static_cast<long long>(sizeof(S));
+            const uint32_t off = c.lo + pz + static_cast<uint32_t>(i) * c.sy;
             if constexpr (VEC) {
-                load_strip_vec<S, V>(reinterpret_cast<const S*>(row + c.xoff), P.v[i]);
+                buf_load<A, V>(c.src, off, P.v[i]);
             } else {
 #pragma unroll
-                for (int j = 0; j < V; ++j)
-                    P.v[i][j] = Storage<S>::load(gld<S>(row + c.xe[j]));
+                for (int j = 0; j < V; ++j) {
+                    A e[1];
+                    buf_load<A, 1>(c.src, off + static_cast<uint32_t>(j) * ES, e);
+                    P.v[i][j] = e[0];
+                }
             }
         }
     }
@@ -81,11 +95,12 @@ struct Stencil3DHL {
         int zero = 0;
         asm volatile("" : "+s"(zero));
         const A* __restrict__ tw = w + zero;
-        const bool zin = q >= 1 && q < c.g.nz - 1;
+        const bool zin = q >= 1 && q < c.nz - 1;
+        const uint32_t qz = static_cast<uint32_t>(q) * c.sz;
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
             const int y = c.y0 + i;
-            const bool yin = zin && y >= 1 && y < c.g.ny - 1;
+            const bool yin = zin && y >= 1 && y < c.ny - 1;
             A out[V];
 #pragma unroll
             for (int j = 0; j < V; ++j) {
@@ -96,32 +111,43 @@ struct Stencil3DHL {
                     const Plane& P = ring[(KT + 1 + tp.dz + NP) % NP];
                     acc = fmaa(tw[t], at(P, i + 1 + tp.dy, j, tp.dx), acc);
                 }
-                const int x = c.x0 + j;
-                out[j] = (yin && x >= 1 && x < c.g.nx - 1) ? acc : A(0);
+                out[j] = (yin && c.xin[j]) ? acc : A(0);
             }
-            if (c.store_lane && y < c.g.ny && q < c.g.nz) {
-                char* row = reinterpret_cast<char*>(c.dst) +
-                            (static_cast<long long>(q) * c.g.sz + static_cast<long long>(y) * c.g.sy) *
-                                static_cast<long long>(sizeof(S));
+            if (y < c.ny) {
+                const uint32_t roff = qz + static_cast<uint32_t>(y) * c.sy;
                 if constexpr (VEC) {
-                    store_strip_vec<S, V>(reinterpret_cast<S*>(row + c.xoff), out);
+                    buf_store<A, V>(c.dst, c.so[0] + roff, out);
                 } else {
 #pragma unroll
-                    for (int j = 0; j < V; ++j)
-                        if (c.x0 + j < c.g.nx)
-                            gst<S>(row + static_cast<uint32_t>((c.x0 + j) * sizeof(S)),
-                                Storage<S>::store(out[j]));
+                    for (int j = 0; j < V; ++j) {
+                        const A e[1] = {out[j]};
+                        buf_store<A, 1>(c.dst, c.so[j] + roff, e);
+                    }
                 }
             }
         }
     }
 
+    // A full ring period (NP ticks) as straight-line code: the compiler's memory
+    // counter waits stay exact, so prefetched planes stay in flight across it.
     template <bool VEC, int KT>
     __device__ static __forceinline__ void period(const Ctx& c, const A* __restrict__ w, int zq0,
-                                                  int zend, Plane (&ring)[NP]) {
+                                                  Plane (&ring)[NP]) {
         if constexpr (KT < NP) {
-            if (zq0 + KT < zend) tick<VEC, KT>(c, w, zq0, ring);
-            period<VEC, KT + 1>(c, w, zq0, zend, ring);
+            tick<VEC, KT>(c, w, zq0, ring);
+            period<VEC, KT + 1>(c, w, zq0, ring);
+        }
+    }
+
+    // the last partial period: ticks zq0+KT < zend only
+    template <bool VEC, int KT>
+    __device__ static __forceinline__ void tail(const Ctx& c, const A* __restrict__ w, int zq0, int zend,
+                                                Plane (&ring)[NP]) {
+        if constexpr (KT < NP - 1) {
+            if (zq0 + KT < zend) {
+                tick<VEC, KT>(c, w, zq0, ring);
+                tail<VEC, KT + 1>(c, w, zq0, zend, ring);
+            }
         }
     }
 
@@ -131,29 +157,38 @@ struct Stencil3DHL {
         // planes zs-1 .. zs+PF into the slots of the first period
 #pragma unroll
         for (int k = 0; k < 2 + PF; ++k) load_plane<VEC>(c, zs - 1 + k, ring[k]);
-        for (int zq0 = zs; zq0 < ze; zq0 += NP) period<VEC, 0>(c, w, zq0, ze, ring);
+        int zq0 = zs;
+#pragma nounroll
+        for (; zq0 + NP <= ze; zq0 += NP) period<VEC, 0>(c, w, zq0, ring);
+        tail<VEC, 0>(c, w, zq0, ze, ring);
     }
 
-    // One tile: strip `strip`, y-block `yb`, output planes [zs, ze) (wave-uniform).
+    // One tile: strip `strip`, y-block `yb`, output planes [zs, ze) (wave-uniform)
+    // of a dense grid (sy = nx, sz = nx*ny) that fits grid3d_fits_buffer.
     // vec: nx % V == 0 and src/dst aligned to V elements.
     __device__ static __forceinline__ void run_tile(const S* __restrict__ src, S* __restrict__ dst,
                                                     const Grid3D& g, bool vec, const A* __restrict__ w,
                                                     int strip, int yb, int zs, int ze) {
         const int lane = threadIdx.x & (kWave - 1);
+        const uint32_t bytes = static_cast<uint32_t>(g.nx) * static_cast<uint32_t>(g.ny) *
+                               static_cast<uint32_t>(g.nz) * ES;
         Ctx c;
-        c.src = src;
-        c.dst = dst;
-        c.g = g;
-        c.x0 = strip * OW + (lane - 1) * V;
+        c.src = buf_rsrc(src, bytes);
+        c.dst = buf_rsrc(dst, bytes);
+        c.ny = g.ny;
+        c.nz = g.nz;
+        c.sy = static_cast<uint32_t>(g.sy) * ES;
+        c.sz = static_cast<uint32_t>(g.sz) * ES;
         c.y0 = yb * NY;
-        c.store_lane = lane >= 1 && lane < kWave - 1 && c.x0 < g.nx;
-        // a lane whose strip leaves the grid reads the last aligned strip (its
-        // values reach only lanes that store nothing or boundary columns)
-        const int xv = max(min(c.x0, g.nx - V), 0);
-        c.xoff = static_cast<uint32_t>(xv) * static_cast<uint32_t>(sizeof(S));
+        const int x0 = strip * OW + (lane - 1) * V;
+        c.lo = static_cast<uint32_t>(x0) * ES;  // x0 = -V wraps: reads past the end
+        const bool store_lane = lane >= 1 && lane < kWave - 1;
 #pragma unroll
-        for (int j = 0; j < V; ++j)
-            c.xe[j] = static_cast<uint32_t>(clampi(c.x0 + j, 0, g.nx - 1)) * static_cast<uint32_t>(sizeof(S));
+        for (int j = 0; j < V; ++j) {
+            const int x = x0 + j;
+            c.so[j] = (store_lane && x < g.nx) ? c.lo + static_cast<uint32_t>(j) * ES : kBufOOB;
+            c.xin[j] = x >= 1 && x < g.nx - 1;
+        }
         if (V == 1 || vec) stream<true>(c, w, zs, ze);
         else stream<false>(c, w, zs, ze);
     }

@@ -30,7 +30,8 @@ def _check2d(got, x, f, tol):
 
 def _mixed_items(storage):
     specs2d = [(37, 251, "star", 1), (300, 129, "box", 2), (64, 64, "box", 5), (1, 7, "star", 1),
-               (250, 997, "box", 1), (513, 300, "box", 5), (45, 600, "star", 3), (129, 130, "box", 2)]
+               (250, 997, "box", 1), (513, 300, "box", 5), (45, 600, "star", 3), (129, 130, "box", 2),
+               (96, 200, "box", 2)]
     items, refs = [], []
     for k, (h, w, shape, r) in enumerate(specs2d):
         src, x = _mk2d(h, w, 100 + k, pad=(k % 3) * 4, dtype=storage)
@@ -69,12 +70,20 @@ def _check_refs(refs, tol):
             assert e["max_rel"] <= 3e-5, e
 
 
-@pytest.mark.parametrize("storage,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2)])
-def test_batch_mixed_2d_3d(storage, tol):
+@pytest.mark.parametrize("split,mfma", [("1", "1"), ("0", "1"), ("1", "0"), ("0", "0")])
+@pytest.mark.parametrize("storage,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
+def test_batch_mixed_2d_3d(storage, tol, split, mfma, monkeypatch):
+    """Both 2D launch forms (one launch per kind / one for all kinds), with and
+    without the matrix-core 5x5 / 11x11 boxes (BF16/FP16 rows of a multiple of 8
+    elements), against the oracle on every replay."""
+    monkeypatch.setenv("SSAM_BATCH2D_SPLIT", split)
+    monkeypatch.setenv("SSAM_BATCH2D_MFMA", mfma)
     items, refs = _mixed_items(storage)
     plan = api.BatchPlan(items)
     t2, t3, pts = plan.info()
     assert t2 > 0 and t3 > 0 and pts > 0
+    t = plan.tiling()
+    assert (t["split_2d"], t["mfma_2d"]) == (int(split), int(mfma))
     for rep in range(3):
         for _, dst, _, _ in refs:
             dst.fill_(float("nan"))
@@ -107,8 +116,10 @@ def test_batch_tune_keeps_results():
     plan = api.BatchPlan(items)
     t = plan.tune()
     assert set(t["seg_rows_2d"].values()) <= {32, 64, 128, 256}
-    assert t["variant_3d"] in (2, 3, 4) and t["zseg_3d"] in (16, 32)
+    assert 0 <= t["variant_3d"] < 8 and t["zseg_3d"] in (16, 32, 64)
     assert t["affinity_2d"] in (0, 1) and t["affinity_3d"] in (0, 1)
+    assert t["split_2d"] in (0, 1) and t["mfma_2d"] in (0, 1)
+    assert t["wps_2d"] in (0, 1, 2, 4) and t["wps_3d"] in (1, 2, 4)
     for rep in range(2):
         for _, dst, _, _ in refs:
             dst.fill_(float("nan"))
@@ -137,9 +148,12 @@ _SIZES3 = [(70, 33, 20), (1, 1, 1), (2, 3, 2), (3, 1, 5), (63, 5, 4), (62, 2, 3)
 def test_batch3d_tile_shapes(variant, affinity, storage, monkeypatch):
     """Every 3D tile shape of the batched launch (columns / rows per lane, prefetch
     depth) on tiny, odd-width and multi-strip grids, short z segments, two step
-    counts (two groups), graph replay; against the oracle."""
+    counts (two groups), persistent grids of 1, 2 and all waves per SIMD, graph
+    replay; against the oracle."""
+    monkeypatch.setenv("SSAM_BATCH3D_WPS", str(variant % 3))
     monkeypatch.setenv("SSAM_BATCH3D_VARIANT", str(variant))
     monkeypatch.setenv("SSAM_BATCH3D_AFFINITY", str(affinity))
+    monkeypatch.setenv("SSAM_BATCH3D_ALTERNATE", str(variant // 4))
     monkeypatch.setenv("SSAM_BATCH3D_ZSEG", "5")
     items, refs = [], []
     for k, (nx, ny, nz) in enumerate(_SIZES3):
