@@ -228,22 +228,24 @@ int ssam_batch_run(ssam_batch_plan plan, void* stream, uint32_t flags);
 int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles_3d,
                          int64_t* point_updates);
 /* Autotune the plan's tiling on its own buffers (dst and work are overwritten;
- * the stream is synchronised): the 2D launch form (one launch per stencil kind,
- * or one for all) and whether BF16/FP16 5x5 and 11x11 boxes run on the matrix
- * cores, the 2D tile height per shape (32..256 rows), the 3D tile shape (columns
- * and rows per lane, prefetch depth) and z-segment length, the persistent grid's
- * waves per SIMD and the queue affinity of each, and the 3D sweep direction,
- * timed over eager runs; the fastest is kept and the next ssam_batch_run
- * captures a new graph. */
+ * the stream is synchronised): the 2D launch forms (one launch per stencil kind
+ * or one for all; persistent or grid) and whether BF16/FP16 5x5 and 11x11 boxes
+ * run on the matrix cores, the 2D tile height per shape (32..256 rows), the 3D
+ * tile shape (columns and rows per lane, prefetch depth), z-segment length and
+ * launch form, the persistent grid's waves per SIMD and the queue affinity of
+ * each, and the 3D sweep direction, timed over eager runs; the fastest is kept
+ * and the next ssam_batch_run captures a new graph. */
 int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
 /* The plan's tiling: 2D tile height per shape (4 entries: 5-point star, 3x3,
- * 5x5, 11x11 box), 3D tile-shape index and z-segment length, and 7 scheduling
+ * 5x5, 11x11 box), 3D tile-shape index and z-segment length, and 9 scheduling
  * entries: queue affinity of the 2D and of the 3D launches (1: waves start at
  * their own XCD's queue, 0: all waves work through the tiles in table order),
  * whether 3D sweeps alternate direction (odd steps walk the table back to front),
  * the 2D launch form (1: one persistent launch per stencil kind, 0: one for all
- * kinds), the matrix-core routing of BF16/FP16 boxes (1: on), and the waves per
- * SIMD of the 2D and of the 3D persistent grids (0: as many as fit).  Any
+ * kinds), the matrix-core routing of BF16/FP16 boxes (1: on), the waves per
+ * SIMD of the 2D and of the 3D persistent grids (0: as many as fit), and the 3D
+ * and 2D launch forms (1: an ordinary grid with one tile per wave, the hardware's
+ * workgroup dispatch taking the place of the queues; 0: persistent).  Any
  * pointer may be NULL. */
 int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* variant_3d,
                            int32_t* zseg_3d, int32_t* sched);

@@ -296,13 +296,14 @@ class BatchPlan:
 
     def tiling(self) -> dict:
         seg = (ctypes.c_int32 * 4)()
-        aff = (ctypes.c_int32 * 7)()
+        aff = (ctypes.c_int32 * 9)()
         var, zseg = ctypes.c_int32(), ctypes.c_int32()
         check(L.lib().ssam_batch_plan_tiling(self._h, seg, ctypes.byref(var), ctypes.byref(zseg), aff),
               "ssam_batch_plan_tiling")
         return dict(seg_rows_2d=dict(zip(("star1", "box1", "box2", "box5"), list(seg))),
                     variant_3d=var.value, zseg_3d=zseg.value, affinity_2d=aff[0], affinity_3d=aff[1],
-                    alternate_3d=aff[2], split_2d=aff[3], mfma_2d=aff[4], wps_2d=aff[5], wps_3d=aff[6])
+                    alternate_3d=aff[2], split_2d=aff[3], mfma_2d=aff[4], wps_2d=aff[5], wps_3d=aff[6],
+                    grid_3d=aff[7], grid_2d=aff[8])
 
     def info(self) -> tuple[int, int, int]:
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

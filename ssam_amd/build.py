@@ -43,6 +43,8 @@ SOURCES = [
     "stencil2d_f16.hip",
     "stencil3d.hip",
     "batch.hip",
+    "batch2d.hip",
+    "batch3d.hip",
     "microbench.hip",
 ]
 

@@ -30,236 +30,9 @@
 #include <new>
 #include <vector>
 
-#include "ssam_internal.hpp"
-#include "stencil2d_dispatch.hpp"
-#include "stencil3d.hpp"
-#include "stencil3d_hl.hpp"
+#include "batch_common.hpp"
 
 namespace ssam {
-
-// ------------------------------------------------------------------ 2D menu
-// Shapes the persistent 2D kernel runs in one launch (mixes of 5-point stars and
-// 3x3 / 5x5 / 11x11 boxes); other shapes fall back to their own launch.
-// k2Box2M / k2Box5M: the 5x5 / 11x11 box on the matrix cores (BF16/FP16 storage).
-enum Kind2D : int { k2Star1 = 0, k2Box1 = 1, k2Box2 = 2, k2Box5 = 3, k2Box5M = 4, k2Box2M = 5, k2None = -1 };
-constexpr int kNumKind2D = 4;   // tile-height slots (k2Box2M / k2Box5M use k2Box2's / k2Box5's)
-constexpr int kNumLaunch2D = 6;  // launch kinds of the split form (one persistent launch each)
-
-inline int seg_slot(int kind) { return kind == k2Box5M ? k2Box5 : kind == k2Box2M ? k2Box2 : kind; }
-
-inline int kind2d(int shape, int r) {
-    if (shape == kStar && r == 1) return k2Star1;
-    if (shape == kBox && r == 1) return k2Box1;
-    if (shape == kBox && r == 2) return k2Box2;
-    if (shape == kBox && r == 5) return k2Box5;
-    return k2None;
-}
-
-template <typename S, int R, int SHAPE> struct Inst2D {
-    static constexpr int V = Lanes2D<S>::V;
-    using UR = Unroll2D<R, SHAPE, V, 1>;
-    using K = Stencil2D<S, R, SHAPE, V, 1, UR::PF, UR::RING, UR::U>;
-    using G = Geom2D<R, V, 1>;
-};
-
-struct BatchItem2D {
-    const void* src;
-    void* dst;
-    Params2D p;
-    ParamsM pm;    // k2Box5M tiles
-    int kind;
-    int taps_off;  // offset into the taps array, in accumulation-type elements
-};
-
-struct BatchItem3D {
-    const void* src;
-    void* dst;
-    void* work;
-    Grid3D g;
-    int nstrips, nyb;
-    int vec;       // nx % V == 0 and every buffer aligned to V elements
-    int steps;
-    int taps_off;
-};
-
-constexpr int kBatchWavesPerBlock = 4;
-constexpr int kQueues = 8;  // one per XCD
-
-// First tile index of each queue (queue q holds tiles [beg[q], beg[q+1])).
-// affinity 1: a wave starts at its own XCD's queue; 0: every wave starts at
-// queue 0, so the whole device works through the tiles in table order.
-// alternate (3D): odd steps walk the table back to front.
-struct QueueSet {
-    int beg[kQueues + 1];
-    int affinity;
-    int alternate;
-};
-
-// Physical XCD of this wave (hardware register XCC_ID, bits 3:0).
-__device__ __forceinline__ int xcd_id() {
-    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kQueues - 1);
-}
-
-// Each queue's counter sits on a 256-byte line of its own: atomics on one
-// address serialise where they execute, atomics on different lines do not.
-constexpr int kCounterStride = 64;                      // ints
-constexpr int kCounterSet = kQueues * kCounterStride;   // ints per launch
-
-// Lane 0 claims the next tile of a queue (one atomic); other lanes return len.
-__device__ __forceinline__ int claim(int* counter, int len) {
-    int t = len;
-    if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(counter, 1);
-    return t;
-}
-
-// The same for a queue that is likely drained already (another XCD's): one
-// relaxed load first, so finding it empty costs no atomic.
-__device__ __forceinline__ int claim_checked(int* counter, int len) {
-    int t = len;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len)
-            t = atomicAdd(counter, 1);
-    }
-    return t;
-}
-
-// Run body(tile) for tiles of this XCD's queue, then of the others in turn,
-// until every queue is empty.  counters: kCounterSet ints, zero at launch.
-// reverse: walk queues and tiles back to front (the 3D sweeps alternate, so a
-// step starts on the grids the previous step wrote last, still in the last-level
-// cache).  AHEAD: the claim of a wave's next tile is issued before its current
-// tile runs, so the atomic's round trip overlaps the tile (for short tiles; with
-// few long tiles per wave, a tile held in reserve by a busy wave lengthens the
-// tail).
-template <bool AHEAD, typename F>
-__device__ __forceinline__ void drain_queues(const QueueSet& qs, int* __restrict__ counters, bool reverse,
-                                             F&& body) {
-    const int x = qs.affinity ? xcd_id() : 0;
-    // one flat loop (a nested queue loop around the tile body costs the tile
-    // registers): k counts the queues already drained
-    int k = 0;
-    int q = reverse ? kQueues - 1 - x : x;
-    int beg = qs.beg[q], len = qs.beg[q + 1] - beg;
-    int n = __builtin_amdgcn_readfirstlane(claim(counters + q * kCounterStride, len));
-    for (;;) {
-        if (n >= len) {
-            if (++k == kQueues) break;
-            const int qq = (x + k) & (kQueues - 1);
-            q = reverse ? kQueues - 1 - qq : qq;
-            beg = qs.beg[q];
-            len = qs.beg[q + 1] - beg;
-            n = __builtin_amdgcn_readfirstlane(claim_checked(counters + q * kCounterStride, len));
-            continue;
-        }
-        if constexpr (AHEAD) {
-            const int next = claim(counters + q * kCounterStride, len);
-            body(reverse ? beg + len - 1 - n : beg + n);
-            n = __builtin_amdgcn_readfirstlane(next);
-        } else {
-            body(reverse ? beg + len - 1 - n : beg + n);
-            n = __builtin_amdgcn_readfirstlane(claim(counters + q * kCounterStride, len));
-        }
-    }
-}
-
-template <typename S> constexpr bool kHalf2D = std::is_same<S, bf16_t>::value || std::is_same<S, f16_t>::value;
-
-// One tile of one launch kind.
-template <typename S, int KIND>
-__device__ __forceinline__ void run_tile2d(const S* src, S* dst, const BatchItem2D& b,
-                                           const typename Storage<S>::acc* w, int strip, int seg) {
-    if constexpr (KIND == k2Star1) Inst2D<S, 1, kStar>::K::run_tile(src, dst, b.p, w, strip, seg);
-    else if constexpr (KIND == k2Box1) Inst2D<S, 1, kBox>::K::run_tile(src, dst, b.p, w, strip, seg);
-    else if constexpr (KIND == k2Box2) Inst2D<S, 2, kBox>::K::run_tile(src, dst, b.p, w, strip, seg);
-    else if constexpr (KIND == k2Box5) Inst2D<S, 5, kBox>::K::run_tile(src, dst, b.p, w, strip, seg);
-    else if constexpr (kHalf2D<S> && KIND == k2Box2M)
-        Mfma2D<S, 2, std::is_same<S, f16_t>::value, 2, 1>::run_tile(src, dst, b.pm, w, strip, seg);
-    else if constexpr (kHalf2D<S> && KIND == k2Box5M)
-        Mfma2D<S, 5, std::is_same<S, f16_t>::value, 2, 1>::run_tile(src, dst, b.pm, w, strip, seg);
-}
-
-// Waves per SIMD the register allocation leaves room for: the matrix-core tiles
-// as their single-grid launch (3 with one set of taps, 2 with hi + lo taps), the
-// rest 2 (the fused form holds the largest tile body of all kinds).
-template <typename S, int KIND> struct Waves2D {
-    static constexpr int value =
-        (KIND == k2Box2M || KIND == k2Box5M) ? (std::is_same<S, f16_t>::value ? 2 : 3) : 2;
-};
-
-// KIND >= 0: every tile of the launch has that kind (the split form, one launch
-// per kind); KIND < 0: the fused form, one launch for all kinds, dispatching per
-// item.
-template <typename S, int KIND>
-__global__ __launch_bounds__(kWave * kBatchWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(Waves2D<S, KIND>::value)))
-void batch2d_kernel(const BatchItem2D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
-                    int* __restrict__ counters, const typename Storage<S>::acc* __restrict__ taps) {
-    drain_queues<true>(qs, counters, false, [&](int t) {
-        const int4 e = tiles[t];
-        const int it = __builtin_amdgcn_readfirstlane(e.x);
-        const int strip = __builtin_amdgcn_readfirstlane(e.y);
-        const int seg = __builtin_amdgcn_readfirstlane(e.z);
-        const BatchItem2D& b = items[it];
-        const S* src = static_cast<const S*>(b.src);
-        S* dst = static_cast<S*>(b.dst);
-        const typename Storage<S>::acc* w = taps + b.taps_off;
-        if constexpr (KIND >= 0) {
-            run_tile2d<S, KIND>(src, dst, b, w, strip, seg);
-        } else {
-            switch (b.kind) {
-                case k2Star1: run_tile2d<S, k2Star1>(src, dst, b, w, strip, seg); break;
-                case k2Box1: run_tile2d<S, k2Box1>(src, dst, b, w, strip, seg); break;
-                case k2Box2: run_tile2d<S, k2Box2>(src, dst, b, w, strip, seg); break;
-                case k2Box2M: run_tile2d<S, k2Box2M>(src, dst, b, w, strip, seg); break;
-                case k2Box5M: run_tile2d<S, k2Box5M>(src, dst, b, w, strip, seg); break;
-                default: run_tile2d<S, k2Box5>(src, dst, b, w, strip, seg); break;
-            }
-        }
-    });
-}
-
-// Batched 3D tile shapes: (columns per lane, rows per lane, planes prefetched).
-// Narrow grids favour few columns per lane (62*V-column strips waste little of a
-// 64..256-column row); more rows per lane cut the y-halo re-reads.
-// wpe: waves per SIMD the register allocation must leave room for (bounds the
-// compiler's load hoisting to what each shape needs).
-struct Var3D { int v, ny, pf, wpe; };
-constexpr Var3D kVar3D[] = {{1, 8, 2, 4}, {2, 16, 1, 2}, {2, 4, 2, 4}, {2, 8, 1, 3},
-                            {2, 8, 2, 2}, {2, 12, 2, 2}, {1, 16, 2, 2}, {4, 8, 1, 2}};
-constexpr int kNumVar3D = 8;
-constexpr int kDefaultVar3D = 4;
-constexpr int kDefaultZseg3D = 32;
-constexpr int kDefaultSeg2D = 64;
-
-template <typename S, int VAR> struct Var3DT {
-    // FP64 runs one column per lane (a plane row of NY+2 doubles per lane)
-    static constexpr int V = sizeof(S) == 8 ? 1 : kVar3D[VAR].v;
-    static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
-    static constexpr int WPE = sizeof(S) == 8 ? (NY >= 16 ? 1 : 2) : kVar3D[VAR].wpe;
-};
-
-// tile.x item, .y strip, .z y-block, .w first plane (low 20 bits) | planes << 20
-template <typename S, int VAR, int SHAPE>
-__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(Var3DT<S, VAR>::WPE)))
-void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
-                    int* __restrict__ counters, const S* __restrict__ taps, int step) {
-    using T = Var3DT<S, VAR>;
-    drain_queues<false>(qs, counters, qs.alternate && (step & 1), [&](int t) {
-        const int4 e = tiles[t];
-        const int it = __builtin_amdgcn_readfirstlane(e.x);
-        const int strip = __builtin_amdgcn_readfirstlane(e.y);
-        const int yb = __builtin_amdgcn_readfirstlane(e.z);
-        const int zw = __builtin_amdgcn_readfirstlane(e.w);
-        const int zs = zw & 0xFFFFF, ze = zs + (zw >> 20);
-        const BatchItem3D& b = items[it];
-        // ping-pong so the last sweep writes dst (same rule as ssam_stencil3d)
-        const void* in = step == 0 ? b.src : (((b.steps - step) % 2 == 0) ? b.dst : b.work);
-        void* out = ((b.steps - 1 - step) % 2 == 0) ? b.dst : b.work;
-        Stencil3DHL<S, SHAPE, T::V, T::NY, T::PF>::run_tile(static_cast<const S*>(in),
-                                                             static_cast<S*>(out), b.g, b.vec != 0,
-                                                             taps + b.taps_off, strip, yb, zs, ze);
-    });
-}
 
 __global__ void reset_counters_kernel(int* __restrict__ c, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -304,6 +77,7 @@ struct ssam_batch_plan_s {
     int split2d = 1;           // 2D: one launch per kind (1) or one for all kinds (0)
     int mfma2d = 1;            // BF16/FP16 5x5 and 11x11 boxes on the matrix cores
     int wps2d = 0, wps3d = 1;  // persistent grid: waves per SIMD (0: as many as fit)
+    int grid2d = 0, grid3d = 0;  // one tile per wave of an ordinary grid (1) or persistent (0)
     int ncounters = 0;
     void* d_counters = nullptr;
     int64_t tiles2d = 0, tiles3d = 0, point_updates = 0;
@@ -346,37 +120,6 @@ int resident_blocks(const void* fn, int wps) {
         per = 1;
     if (wps > 0 && wps < per) per = wps;
     return cus * per;
-}
-
-template <typename S>
-using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, const S*, int);
-
-template <typename S, int SHAPE> Batch3DFn<S> batch3d_fn_shape(int variant) {
-    switch (variant) {
-        case 0: return batch3d_kernel<S, 0, SHAPE>;
-        case 1: return batch3d_kernel<S, 1, SHAPE>;
-        case 2: return batch3d_kernel<S, 2, SHAPE>;
-        case 3: return batch3d_kernel<S, 3, SHAPE>;
-        case 4: return batch3d_kernel<S, 4, SHAPE>;
-        case 5: return batch3d_kernel<S, 5, SHAPE>;
-        case 6: return batch3d_kernel<S, 6, SHAPE>;
-        default: return batch3d_kernel<S, 7, SHAPE>;
-    }
-}
-template <typename S> const void* batch3d_fn(int variant, int kind) {
-    return reinterpret_cast<const void*>(kind ? batch3d_fn_shape<S, k3FaceEdge>(variant)
-                                              : batch3d_fn_shape<S, k3Star>(variant));
-}
-template <typename S> const void* batch2d_fn(int kind) {
-    switch (kind) {
-        case k2Star1: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Star1>);
-        case k2Box1: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box1>);
-        case k2Box2: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box2>);
-        case k2Box5: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box5>);
-        case k2Box2M: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box2M>);
-        case k2Box5M: return reinterpret_cast<const void*>(batch2d_kernel<S, k2Box5M>);
-        default: return reinterpret_cast<const void*>(batch2d_kernel<S, -1>);
-    }
 }
 
 // One item's tiles in memory order with a per-tile cost estimate.
@@ -438,6 +181,22 @@ void pack_queues(const std::vector<ItemTiles>& per_item, std::vector<int4>* tile
     qs->beg[kQueues] = static_cast<int>(tiles->size());
 }
 
+// One list for a grid-mode launch: all tiles by decreasing cost (the hardware
+// dispatches workgroups in order, so the longest tiles start first), each item's
+// tiles of equal cost in memory order.  Every queue bound points at the end.
+void pack_single(const std::vector<ItemTiles>& per_item, std::vector<int4>* tiles, QueueSet* qs) {
+    struct Ref { double cost; int item; int k; };
+    std::vector<Ref> all;
+    for (int i = 0; i < static_cast<int>(per_item.size()); ++i)
+        for (int k = 0; k < static_cast<int>(per_item[i].tiles.size()); ++k)
+            all.push_back({per_item[i].cost[k], i, k});
+    std::stable_sort(all.begin(), all.end(), [](const Ref& a, const Ref& b) { return a.cost > b.cost; });
+    tiles->clear();
+    for (const Ref& r : all) tiles->push_back(per_item[r.item].tiles[r.k]);
+    qs->beg[0] = 0;
+    for (int q = 1; q <= kQueues; ++q) qs->beg[q] = static_cast<int>(tiles->size());
+}
+
 template <typename T>
 int upload(void** dptr, const std::vector<T>& h) {
     if (hipMalloc(dptr, std::max<size_t>(1, h.size()) * sizeof(T)) != hipSuccess) {
@@ -458,8 +217,8 @@ void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p,
     p->nstrips = static_cast<int>((it.nx + I::G::OW - 1) / I::G::OW);
     p->seg_rows = seg_rows;
     p->boundary = kZero;
-    p->vec_ok = (it.pitch % I::V == 0) && (reinterpret_cast<uintptr_t>(it.src) % 16 == 0) &&
-                (reinterpret_cast<uintptr_t>(it.dst) % 16 == 0);
+    p->vec_ok = (reinterpret_cast<uintptr_t>(it.src) % sizeof(S) == 0) &&
+                (reinterpret_cast<uintptr_t>(it.dst) % sizeof(S) == 0);
     const int nsegs = (p->height + seg_rows - 1) / seg_rows;
     p->ntiles = p->nstrips * nsegs;
     p->nblocks = 0;
@@ -530,10 +289,9 @@ int build_group2d(Plan* P, Plan::Group& g) {
             }
         const int idx = static_cast<int>(k);
         const int seg = P->seg2d[b.kind];
-        // 5x5 and 11x11 boxes in BF16/FP16 go to the matrix cores when rows are
-        // 16-byte aligned
-        if (P->mfma2d && (b.kind == k2Box2 || b.kind == k2Box5) && kHalf2D<S> && it.pitch % 8 == 0 &&
-            ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 16) == 0) {
+        // 5x5 and 11x11 boxes in BF16/FP16 go to the matrix cores
+        if (P->mfma2d && (b.kind == k2Box2 || b.kind == k2Box5) && kHalf2D<S> &&
+            ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 2) == 0) {
             b.kind = b.kind == k2Box2 ? k2Box2M : k2Box5M;
             tiles_for_mfma(it, idx, R, seg, &b.pm, &per[k]);
             continue;
@@ -560,12 +318,14 @@ int build_group2d(Plan* P, Plan::Group& g) {
         Plan::Group::Sub sb;
         sb.kind = P->split2d ? kd : -1;
         std::vector<int4> t;
-        pack_queues(part, &t, &sb.qs);
+        if (P->grid2d) pack_single(part, &t, &sb.qs);
+        else pack_queues(part, &t, &sb.qs);
         sb.qs.affinity = P->aff2d;
         sb.qs.alternate = 0;
         sb.tile_off = static_cast<int>(tiles.size());
         sb.ntiles = static_cast<int>(t.size());
-        sb.blocks = resident_blocks(batch2d_fn<S>(sb.kind), P->wps2d);
+        sb.blocks = P->grid2d ? (sb.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock
+                              : resident_blocks(batch2d_kernel_fn(g.storage, sb.kind, false), P->wps2d);
         tiles.insert(tiles.end(), t.begin(), t.end());
         g.subs.push_back(sb);
     }
@@ -621,12 +381,14 @@ int build_group3d(Plan* P, Plan::Group& g) {
         }
     }
     std::vector<int4> tiles;
-    pack_queues(per, &tiles, &g.qs);
+    if (P->grid3d) pack_single(per, &tiles, &g.qs);
+    else pack_queues(per, &tiles, &g.qs);
     g.qs.affinity = P->aff3d;
     g.qs.alternate = P->alt3d;
     free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
-    g.blocks = resident_blocks(batch3d_fn<S>(variant, g.kind), P->wps3d);
+    g.blocks = P->grid3d ? (g.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock
+                         : resident_blocks(batch3d_kernel_fn(g.storage, variant, g.kind, false), P->wps3d);
     int rc = upload(&g.d_items, h);
     if (rc == SSAM_OK) rc = upload(&g.d_tiles, tiles);
     if (rc == SSAM_OK) rc = upload(&g.d_taps, taps);
@@ -697,13 +459,7 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             int* c = cnt + kCounterSet * k;
             const void* taps = g.d_taps;
             void* args[] = {&items, &tiles, &qs, &c, &taps};
-            const void* fn = nullptr;
-            switch (g.storage) {
-                case SSAM_F32: fn = batch2d_fn<float>(sb.kind); break;
-                case SSAM_F64: fn = batch2d_fn<double>(sb.kind); break;
-                case SSAM_BF16: fn = batch2d_fn<bf16_t>(sb.kind); break;
-                default: fn = batch2d_fn<f16_t>(sb.kind); break;
-            }
+            const void* fn = batch2d_kernel_fn(g.storage, sb.kind, P->grid2d != 0);
             if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
                     hipSuccess ||
                 hipGetLastError() != hipSuccess)
@@ -752,8 +508,7 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             int step = s;
             const void* taps = g.d_taps;
             void* args[] = {&items, &tiles, &qs, &cnt, &taps, &step};
-            const void* fn = g.storage == SSAM_F64 ? batch3d_fn<double>(P->var3d, g.kind)
-                                                   : batch3d_fn<float>(P->var3d, g.kind);
+            const void* fn = batch3d_kernel_fn(g.storage, P->var3d, g.kind, P->grid3d != 0);
             if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
                     hipSuccess ||
                 hipGetLastError() != hipSuccess)
@@ -866,6 +621,8 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
     P->mfma2d = env_int("SSAM_BATCH2D_MFMA", 1) != 0;
     P->wps2d = std::max(0, env_int("SSAM_BATCH2D_WPS", 0));
     P->wps3d = std::max(0, env_int("SSAM_BATCH3D_WPS", 1));
+    P->grid2d = env_int("SSAM_BATCH2D_GRID", 0) != 0;
+    P->grid3d = env_int("SSAM_BATCH3D_GRID", 0) != 0;
     P->aff3d = env_int("SSAM_BATCH3D_AFFINITY", 0) != 0;
     P->alt3d = env_int("SSAM_BATCH3D_ALTERNATE", 1) != 0;
     for (int s = 0; s < 4; ++s) {
@@ -980,15 +737,17 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
         // launch form x matrix-core routing x persistent-grid occupancy, then the
         // tile height per shape (one shape at a time, the others held), then
         // queue affinity
-        search({&P->split2d, &P->mfma2d, &P->wps2d}, {{0, 1}, {0, 1}, {1, 2, 4, 0}}, kRun2D);
+        search({&P->split2d, &P->mfma2d, &P->grid2d}, {{0, 1}, {0, 1}, {0, 1}}, kRun2D);
+        if (!P->grid2d) search({&P->wps2d}, {{1, 2, 4, 0}}, kRun2D);
         for (int k = 0; k < kNumKind2D; ++k)
             if (kinds[k]) search({&P->seg2d[k]}, {{32, 64, 128, 256}}, kRun2D);
         search({&P->aff2d}, {{0, 1}}, kRun2D);
     }
     if (has3) {
-        // tile shape x z-segment length x occupancy, then queue affinity and
-        // step direction
-        search({&P->var3d, &P->zseg3d, &P->wps3d}, {{0, 1, 2, 3, 4, 5, 6, 7}, {16, 32, 64}, {1, 2, 4}}, kRun3D);
+        // tile shape x z-segment length x launch form (persistent / grid), then
+        // the persistent grid's occupancy, queue affinity and step direction
+        search({&P->var3d, &P->zseg3d, &P->grid3d}, {{0, 1, 2, 3, 4, 5, 6, 7}, {16, 32, 64}, {0, 1}}, kRun3D);
+        if (!P->grid3d) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
         search({&P->aff3d, &P->alt3d}, {{0, 1}, {0, 1}}, kRun3D);
     }
     const int rc2 = build_all(P);
@@ -1006,6 +765,8 @@ int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* 
         sched[4] = plan->mfma2d;
         sched[5] = plan->wps2d;
         sched[6] = plan->wps3d;
+        sched[7] = plan->grid3d;
+        sched[8] = plan->grid2d;
     }
     if (seg_rows_2d)
         for (int k = 0; k < kNumKind2D; ++k) seg_rows_2d[k] = plan->seg2d[k];

@@ -1,0 +1,171 @@
+// batch_common.hpp -- declarations shared by the batched launches (batch.hip:
+// plan and host side; batch2d.hip / batch3d.hip: the persistent and grid-mode
+// kernels): the 2D launch kinds, the item records, the XCD work queues and the
+// 3D tile shapes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "ssam_internal.hpp"
+#include "stencil2d_dispatch.hpp"
+#include "stencil3d.hpp"
+#include "stencil3d_hl.hpp"
+
+namespace ssam {
+
+// ------------------------------------------------------------------ 2D menu
+// Shapes the persistent 2D kernel runs in one launch (mixes of 5-point stars and
+// 3x3 / 5x5 / 11x11 boxes); other shapes fall back to their own launch.
+// k2Box2M / k2Box5M: the 5x5 / 11x11 box on the matrix cores (BF16/FP16 storage).
+enum Kind2D : int { k2Star1 = 0, k2Box1 = 1, k2Box2 = 2, k2Box5 = 3, k2Box5M = 4, k2Box2M = 5, k2None = -1 };
+constexpr int kNumKind2D = 4;   // tile-height slots (k2Box2M / k2Box5M use k2Box2's / k2Box5's)
+constexpr int kNumLaunch2D = 6;  // launch kinds of the split form (one persistent launch each)
+
+inline int seg_slot(int kind) { return kind == k2Box5M ? k2Box5 : kind == k2Box2M ? k2Box2 : kind; }
+
+inline int kind2d(int shape, int r) {
+    if (shape == kStar && r == 1) return k2Star1;
+    if (shape == kBox && r == 1) return k2Box1;
+    if (shape == kBox && r == 2) return k2Box2;
+    if (shape == kBox && r == 5) return k2Box5;
+    return k2None;
+}
+
+template <typename S, int R, int SHAPE> struct Inst2D {
+    static constexpr int V = Lanes2D<S>::V;
+    using UR = Unroll2D<R, SHAPE, V, 1>;
+    using K = Stencil2D<S, R, SHAPE, V, 1, UR::PF, UR::RING, UR::U>;
+    using G = Geom2D<R, V, 1>;
+};
+
+struct BatchItem2D {
+    const void* src;
+    void* dst;
+    Params2D p;
+    ParamsM pm;    // k2Box5M tiles
+    int kind;
+    int taps_off;  // offset into the taps array, in accumulation-type elements
+};
+
+struct BatchItem3D {
+    const void* src;
+    void* dst;
+    void* work;
+    Grid3D g;
+    int nstrips, nyb;
+    int vec;       // nx % V == 0 and every buffer aligned to V elements
+    int steps;
+    int taps_off;
+};
+
+constexpr int kBatchWavesPerBlock = 4;
+constexpr int kQueues = 8;  // one per XCD
+
+// First tile index of each queue (queue q holds tiles [beg[q], beg[q+1])).
+// affinity 1: a wave starts at its own XCD's queue; 0: every wave starts at
+// queue 0, so the whole device works through the tiles in table order.
+// alternate (3D): odd steps walk the table back to front.
+struct QueueSet {
+    int beg[kQueues + 1];
+    int affinity;
+    int alternate;
+};
+
+// Physical XCD of this wave (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int xcd_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kQueues - 1);
+}
+
+// Each queue's counter sits on a 256-byte line of its own: atomics on one
+// address serialise where they execute, atomics on different lines do not.
+constexpr int kCounterStride = 64;                      // ints
+constexpr int kCounterSet = kQueues * kCounterStride;   // ints per launch
+
+// Lane 0 claims the next tile of a queue (one atomic); other lanes return len.
+__device__ __forceinline__ int claim(int* counter, int len) {
+    int t = len;
+    if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(counter, 1);
+    return t;
+}
+
+// The same for a queue that is likely drained already (another XCD's): one
+// relaxed load first, so finding it empty costs no atomic.
+__device__ __forceinline__ int claim_checked(int* counter, int len) {
+    int t = len;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len)
+            t = atomicAdd(counter, 1);
+    }
+    return t;
+}
+
+// Run body(tile) for tiles of this XCD's queue, then of the others in turn,
+// until every queue is empty.  counters: kCounterSet ints, zero at launch.
+// reverse: walk queues and tiles back to front (the 3D sweeps alternate, so a
+// step starts on the grids the previous step wrote last, still in the last-level
+// cache).  AHEAD: the claim of a wave's next tile is issued before its current
+// tile runs, so the atomic's round trip overlaps the tile (for short tiles; with
+// few long tiles per wave, a tile held in reserve by a busy wave lengthens the
+// tail).
+template <bool AHEAD, typename F>
+__device__ __forceinline__ void drain_queues(const QueueSet& qs, int* __restrict__ counters, bool reverse,
+                                             F&& body) {
+    const int x = qs.affinity ? xcd_id() : 0;
+    // one flat loop (a nested queue loop around the tile body costs the tile
+    // registers): k counts the queues already drained
+    int k = 0;
+    int q = reverse ? kQueues - 1 - x : x;
+    int beg = qs.beg[q], len = qs.beg[q + 1] - beg;
+    int n = __builtin_amdgcn_readfirstlane(claim(counters + q * kCounterStride, len));
+    for (;;) {
+        if (n >= len) {
+            if (++k == kQueues) break;
+            const int qq = (x + k) & (kQueues - 1);
+            q = reverse ? kQueues - 1 - qq : qq;
+            beg = qs.beg[q];
+            len = qs.beg[q + 1] - beg;
+            n = __builtin_amdgcn_readfirstlane(claim_checked(counters + q * kCounterStride, len));
+            continue;
+        }
+        if constexpr (AHEAD) {
+            const int next = claim(counters + q * kCounterStride, len);
+            body(reverse ? beg + len - 1 - n : beg + n);
+            n = __builtin_amdgcn_readfirstlane(next);
+        } else {
+            body(reverse ? beg + len - 1 - n : beg + n);
+            n = __builtin_amdgcn_readfirstlane(claim(counters + q * kCounterStride, len));
+        }
+    }
+}
+
+template <typename S> constexpr bool kHalf2D = std::is_same<S, bf16_t>::value || std::is_same<S, f16_t>::value;
+
+// Batched 3D tile shapes: (columns per lane, rows per lane, planes prefetched).
+// Narrow grids favour few columns per lane (62*V-column strips waste little of a
+// 64..256-column row); more rows per lane cut the y-halo re-reads.
+// wpe: waves per SIMD the register allocation must leave room for (bounds the
+// compiler's load hoisting to what each shape needs).
+struct Var3D { int v, ny, pf, wpe; };
+constexpr Var3D kVar3D[] = {{1, 8, 2, 4}, {2, 16, 1, 2}, {2, 4, 2, 4}, {2, 8, 1, 3},
+                            {2, 8, 2, 2}, {2, 12, 2, 2}, {1, 16, 2, 2}, {4, 8, 1, 2}};
+constexpr int kNumVar3D = 8;
+constexpr int kDefaultVar3D = 4;
+constexpr int kDefaultZseg3D = 32;
+constexpr int kDefaultSeg2D = 64;
+
+template <typename S, int VAR> struct Var3DT {
+    // FP64 runs one column per lane (a plane row of NY+2 doubles per lane)
+    static constexpr int V = sizeof(S) == 8 ? 1 : kVar3D[VAR].v;
+    static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
+    static constexpr int WPE = sizeof(S) == 8 ? (NY >= 16 ? 1 : 2) : kVar3D[VAR].wpe;
+};
+
+// Kernel entry points by launch parameters (host stubs for hipLaunchKernel and
+// the occupancy query).  storage: SSAM_F32 / F64 / BF16 / F16; kind: a Kind2D
+// (split form) or -1 (fused); shape3d: 0 7-point star, 1 19-point face+edge;
+// grid: one tile per wave of an ordinary grid instead of a persistent grid.
+const void* batch2d_kernel_fn(int storage, int kind, bool grid);
+const void* batch3d_kernel_fn(int storage, int variant, int shape3d, bool grid);
+
+}  // namespace ssam

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ssam {
 
 constexpr int kWave = 64;
@@ -151,12 +153,16 @@ __device__ __forceinline__ void buf_store(BufRsrc r, uint32_t off, const T (&in)
 }
 
 // ---------------------------------------------------------------- strip I/O
-// One lane's V consecutive elements of a row.  `full` = the whole strip lies
-// inside [0, width) and is aligned for a single vector access.
+// One lane's V consecutive elements of a row, as one vector access.  The vector
+// type only promises the element's alignment: rows of any pitch take the same
+// 8/16-byte access (gfx9-family global memory runs unaligned accesses, in the
+// unaligned mode the ROCm runtime sets), so only the element alignment of the
+// base pointer is required.
 template <typename S, int V> struct VecBits {
     static constexpr int bytes = V * static_cast<int>(sizeof(S));
     static_assert(bytes % 4 == 0, "strip must be a whole number of dwords");
-    using type = uint32_t __attribute__((ext_vector_type(bytes / 4)));
+    typedef uint32_t vec_t __attribute__((ext_vector_type(bytes / 4)));
+    typedef vec_t type __attribute__((aligned(sizeof(S))));
 };
 
 template <typename S, int V>
@@ -179,6 +185,63 @@ __device__ __forceinline__ void store_strip_vec(S* __restrict__ p, const typenam
     __builtin_memcpy(&raw, tmp, sizeof(tmp));
     gst<VB>(p, raw);
 }
+
+// One lane's V elements of a row as the 2D input ring holds them: F32 / F64 as
+// values; BF16 / F16 packed two to a dword as loaded, converted to FP32 where a
+// tick reads them.  A ring slot loaded in one ring period and first read in the
+// next then crosses the loop back-edge as the load left it, so the memory-counter
+// wait sits at the read (exact) instead of at the back-edge, where converting
+// would wait for every row still in flight once per period.
+template <typename S, int V> struct RowReg {
+    using A = typename Storage<S>::acc;
+    static constexpr bool kPacked = sizeof(S) == 2;
+    static constexpr int N = kPacked ? V / 2 : V;
+    using E = typename std::conditional<kPacked, uint32_t, A>::type;
+    static_assert(!kPacked || V % 2 == 0, "packed rows hold element pairs");
+    E r[N];
+
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int n = 0; n < N; ++n) r[n] = E(0);
+    }
+    // one vector access (element-aligned p)
+    __device__ __forceinline__ void load_vec(const S* p) {
+        if constexpr (kPacked) {
+            using VB = typename VecBits<S, V>::type;
+            const VB raw = gld<VB>(p);
+            __builtin_memcpy(r, &raw, sizeof(r));
+        } else {
+            load_strip_vec<S, V>(p, r);
+        }
+    }
+    // from V elements in storage form (edge paths)
+    __device__ __forceinline__ void set(const S (&e)[V]) {
+        if constexpr (kPacked) {
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+                uint16_t lo, hi;
+                __builtin_memcpy(&lo, &e[2 * n], 2);
+                __builtin_memcpy(&hi, &e[2 * n + 1], 2);
+                r[n] = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < V; ++j) r[j] = Storage<S>::load(e[j]);
+        }
+    }
+    __device__ __forceinline__ A get(int j) const {
+        if constexpr (!kPacked) {
+            return r[j];
+        } else if constexpr (std::is_same<S, bf16_t>::value) {
+            return __uint_as_float((j & 1) ? (r[j / 2] & 0xffff0000u) : (r[j / 2] << 16));
+        } else {
+            const uint16_t b = static_cast<uint16_t>((j & 1) ? (r[j / 2] >> 16) : (r[j / 2] & 0xffffu));
+            _Float16 h;
+            __builtin_memcpy(&h, &b, 2);
+            return static_cast<float>(h);
+        }
+    }
+};
 
 // Boundary rules for reads outside the grid.
 enum Boundary : int { kZero = 0, kClamp = 1, kHold = 2, kDirichlet0 = 3 };

@@ -186,6 +186,7 @@ struct Stencil2D {
     static constexpr int NSTAGE_RINGS = T > 1 ? T - 1 : 1;
     static_assert(!RING || (U % NR0 == 0 && (T == 1 || U % NRS == 0)), "ring period");
 
+    using Row = RowReg<S, V>;  // an input ring slot
     static constexpr int s0(int k, int n) { return RING ? (k + n) % NR0 : k + n; }
     static constexpr int sS(int k, int n) { return RING ? (k + n) % NRS : k + n; }
 
@@ -210,36 +211,36 @@ struct Stencil2D {
                static_cast<long long>(y) * L.p.pitch * static_cast<long long>(sizeof(S));
     }
     template <bool CHK>
-    __device__ static __forceinline__ void load_row(const Lane& L, int y, A (&dst)[V]) {
+    __device__ static __forceinline__ void load_row(const Lane& L, int y, Row& dst) {
         if constexpr (!CHK) {
-            load_strip_vec<S, V>(reinterpret_cast<const S*>(row_ptr(L, y) + L.xoff), dst);
+            dst.load_vec(reinterpret_cast<const S*>(row_ptr(L, y) + L.xoff));
         } else {
             int yy = y;
             const bool clamp = L.p.boundary == kClamp;
             if (clamp) {
                 yy = clampi(y, 0, L.p.height - 1);
             } else if (y < 0 || y >= L.p.height) {
-#pragma unroll
-                for (int j = 0; j < V; ++j) dst[j] = 0;
+                dst.zero();
                 return;
             }
             const char* row = row_ptr(L, yy);
             if (L.lane_full && L.p.vec_ok) {
-                load_strip_vec<S, V>(reinterpret_cast<const S*>(row + L.xoff), dst);
+                dst.load_vec(reinterpret_cast<const S*>(row + L.xoff));
             } else {
+                S e[V];
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
                     const int x = L.x0 + j;
                     if (clamp) {
                         const uint32_t o = static_cast<uint32_t>(clampi(x, 0, L.p.width - 1)) *
                                            static_cast<uint32_t>(sizeof(S));
-                        dst[j] = Storage<S>::load(gld<S>(row + o));
+                        e[j] = gld<S>(row + o);
                     } else {
                         const uint32_t o = L.xoff + static_cast<uint32_t>(j * sizeof(S));
-                        dst[j] = L.inb[j] ? Storage<S>::load(gld<S>(row + o))
-                                          : A(0);
+                        e[j] = L.inb[j] ? gld<S>(row + o) : S{};
                     }
                 }
+                dst.set(e);
             }
         }
     }
@@ -262,7 +263,7 @@ struct Stencil2D {
 
     template <bool CHK, int KT>
     __device__ static __forceinline__ void tick(const Lane& L, const A* __restrict__ w, int yc0,
-                                                A (&ring0)[NR0][V],
+                                                Row (&ring0)[NR0],
                                                 A (&rings)[NSTAGE_RINGS][NRS][V]) {
         const int yc = yc0 + KT;
         // rows past the tile's last needed row only feed discarded ticks: re-read
@@ -281,7 +282,7 @@ struct Stencil2D {
             for (int n = 0; n < K; ++n)
 #pragma unroll
                 for (int j = 0; j < V; ++j)
-                    X[n][j] = (t == 1) ? ring0[s0(KT, n)][j] : rings[t - 2][sS(KT, n)][j];
+                    X[n][j] = (t == 1) ? ring0[s0(KT, n)].get(j) : rings[t - 2][sS(KT, n)][j];
             A out[V];
             stencil_row<A, R, SHAPE, V>(X, tw, out);
             const int q = yc - t * R;  // grid row produced by this step
@@ -302,13 +303,13 @@ struct Stencil2D {
                 if constexpr (CHK && T == 1) {
                     if (L.p.boundary == kHold) {
                         // border cells keep their input value
-                        const A* in = ring0[s0(KT, R)];
+                        const Row& in = ring0[s0(KT, R)];
                         const bool row_border = q < R || q >= L.p.height - R;
 #pragma unroll
                         for (int j = 0; j < V; ++j) {
                             const int x = L.x0 + j;
                             const bool border = row_border || x < R || x >= L.p.width - R;
-                            out[j] = border ? in[j] : out[j];
+                            out[j] = border ? in.get(j) : out[j];
                         }
                     }
                 }
@@ -319,7 +320,7 @@ struct Stencil2D {
 
     template <bool CHK, int KT>
     __device__ static __forceinline__ void block(const Lane& L, const A* __restrict__ w, int yc0,
-                                                 A (&ring0)[NR0][V],
+                                                 Row (&ring0)[NR0],
                                                  A (&rings)[NSTAGE_RINGS][NRS][V]) {
         if constexpr (KT < U) {
             tick<CHK, KT>(L, w, yc0, ring0, rings);
@@ -330,7 +331,7 @@ struct Stencil2D {
     template <bool CHK>
     __device__ static __forceinline__ void stream(const Lane& L, const A* __restrict__ w, int c0,
                                                   int nblk) {
-        A ring0[NR0][V];
+        Row ring0[NR0];
         A rings[NSTAGE_RINGS][NRS][V];
 #pragma unroll
         for (int s = 0; s < NSTAGE_RINGS; ++s)
@@ -339,9 +340,7 @@ struct Stencil2D {
 #pragma unroll
                 for (int j = 0; j < V; ++j) rings[s][i][j] = 0;
 #pragma unroll
-        for (int i = 0; i < NR0; ++i)
-#pragma unroll
-            for (int j = 0; j < V; ++j) ring0[i][j] = 0;
+        for (int i = 0; i < NR0; ++i) ring0[i].zero();
         // rows c0-2R .. c0+PF-1 into slots 0 .. 2R+PF-1
 #pragma unroll
         for (int s = 0; s < 2 * R + PF; ++s) load_row<CHK>(L, min(c0 - 2 * R + s, L.ylast), ring0[s]);
@@ -355,9 +354,7 @@ struct Stencil2D {
             block<CHK, 0>(L, w, yc0, ring0, rings);
             if constexpr (!RING) {
 #pragma unroll
-                for (int i = 0; i < 2 * R + PF; ++i)
-#pragma unroll
-                    for (int j = 0; j < V; ++j) ring0[i][j] = ring0[i + U][j];
+                for (int i = 0; i < 2 * R + PF; ++i) ring0[i] = ring0[i + U];
 #pragma unroll
                 for (int s = 0; s < NSTAGE_RINGS; ++s)
 #pragma unroll

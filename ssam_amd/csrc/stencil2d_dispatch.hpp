@@ -58,9 +58,9 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     p.pitch = rq.pitch;
     p.nstrips = static_cast<int>((rq.width + G::OW - 1) / G::OW);
     p.boundary = rq.boundary;
-    const uintptr_t align = static_cast<uintptr_t>(V * sizeof(S) >= 16 ? 16 : V * sizeof(S));
-    p.vec_ok = (rq.pitch % V == 0) && (reinterpret_cast<uintptr_t>(rq.src) % align == 0) &&
-               (reinterpret_cast<uintptr_t>(rq.dst) % align == 0);
+    // one vector access per strip needs only element-aligned rows (any pitch)
+    p.vec_ok = (reinterpret_cast<uintptr_t>(rq.src) % sizeof(S) == 0) &&
+               (reinterpret_cast<uintptr_t>(rq.dst) % sizeof(S) == 0);
     // rows per tile: enough tiles for ~target waves per SIMD, but at least a few
     // multiples of the halo so the 2*T*r re-read rows stay a small fraction
     int seg = rq.seg_rows;

@@ -37,6 +37,9 @@
 namespace ssam {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// 8 consecutive 2-byte elements of a row of any pitch (element alignment only;
+// the access is one unaligned 16-byte load)
+typedef u32x4 u32x4_row __attribute__((aligned(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <typename S> struct MfmaOp;
@@ -120,14 +123,14 @@ struct Mfma2D {
         const int c = L.x0 + 16 * (a / NC) - XS + 32 * (a % NC) + 8 * L.g;
         const char* row = L.src + static_cast<long long>(r) * L.p.pitch * static_cast<long long>(sizeof(S));
         if constexpr (!CHK) {
-            return gld<u32x4>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
+            return gld<u32x4_row>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
         } else {
             const bool rok = r >= 0 && r < L.p.height;
             const bool inside = c >= 0 && c + 8 <= L.p.width;
             const void* a = (rok && inside)
                                 ? static_cast<const void*>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)))
                                 : static_cast<const void*>(kZeroChunk);
-            u32x4 v = gld<u32x4>(a);
+            u32x4 v = gld<u32x4_row>(a);
             if (rok && !inside && c < L.p.width && c + 8 > 0) {
                 uint32_t h[8];
 #pragma unroll
@@ -369,7 +372,7 @@ template <typename S> bool taps_exact(const Req2D& rq) {
 }
 
 // Matrix-core form: BF16/FP16 storage, box/dense radius 1..8 (BF16: 9), one
-// sweep, zero boundary, rows 16-byte aligned.
+// sweep, zero boundary; rows of any pitch.
 template <typename S> bool mfma_eligible(const Req2D& rq) {
     if constexpr (!std::is_same<S, bf16_t>::value && !std::is_same<S, f16_t>::value) {
         (void)rq;
@@ -380,7 +383,7 @@ template <typename S> bool mfma_eligible(const Req2D& rq) {
         const uintptr_t a = reinterpret_cast<uintptr_t>(rq.src) | reinterpret_cast<uintptr_t>(rq.dst);
         const int rmax = std::is_same<S, bf16_t>::value ? 9 : 8;
         return rq.shape == kBox && rq.steps == 1 && rq.boundary == kZero && rq.radius >= 1 &&
-               rq.radius <= rmax && rq.pitch % 8 == 0 && a % 16 == 0;
+               rq.radius <= rmax && a % 2 == 0;
     }
 }
 

@@ -70,20 +70,24 @@ def _check_refs(refs, tol):
             assert e["max_rel"] <= 3e-5, e
 
 
-@pytest.mark.parametrize("split,mfma", [("1", "1"), ("0", "1"), ("1", "0"), ("0", "0")])
+@pytest.mark.parametrize("split,mfma,grid", [("1", "1", "0"), ("0", "1", "0"), ("1", "0", "1"), ("0", "0", "1"),
+                                             ("1", "1", "1")])
 @pytest.mark.parametrize("storage,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
-def test_batch_mixed_2d_3d(storage, tol, split, mfma, monkeypatch):
-    """Both 2D launch forms (one launch per kind / one for all kinds), with and
-    without the matrix-core 5x5 / 11x11 boxes (BF16/FP16 rows of a multiple of 8
-    elements), against the oracle on every replay."""
+def test_batch_mixed_2d_3d(storage, tol, split, mfma, grid, monkeypatch):
+    """The 2D launch forms (one launch per kind / one for all kinds; persistent /
+    one tile per wave of an ordinary grid), with and without the matrix-core 5x5 /
+    11x11 boxes (BF16/FP16 rows of a multiple of 8 elements), against the oracle
+    on every replay."""
     monkeypatch.setenv("SSAM_BATCH2D_SPLIT", split)
     monkeypatch.setenv("SSAM_BATCH2D_MFMA", mfma)
+    monkeypatch.setenv("SSAM_BATCH2D_GRID", grid)
+    monkeypatch.setenv("SSAM_BATCH3D_GRID", grid)
     items, refs = _mixed_items(storage)
     plan = api.BatchPlan(items)
     t2, t3, pts = plan.info()
     assert t2 > 0 and t3 > 0 and pts > 0
     t = plan.tiling()
-    assert (t["split_2d"], t["mfma_2d"]) == (int(split), int(mfma))
+    assert (t["split_2d"], t["mfma_2d"], t["grid_2d"], t["grid_3d"]) == (int(split), int(mfma), int(grid), int(grid))
     for rep in range(3):
         for _, dst, _, _ in refs:
             dst.fill_(float("nan"))
@@ -119,7 +123,8 @@ def test_batch_tune_keeps_results():
     assert 0 <= t["variant_3d"] < 8 and t["zseg_3d"] in (16, 32, 64)
     assert t["affinity_2d"] in (0, 1) and t["affinity_3d"] in (0, 1)
     assert t["split_2d"] in (0, 1) and t["mfma_2d"] in (0, 1)
-    assert t["wps_2d"] in (0, 1, 2, 4) and t["wps_3d"] in (1, 2, 4)
+    assert t["wps_2d"] in (0, 1, 2, 4) and t["wps_3d"] in (1, 2, 4) and t["grid_3d"] in (0, 1)
+    assert t["grid_2d"] in (0, 1)
     for rep in range(2):
         for _, dst, _, _ in refs:
             dst.fill_(float("nan"))
@@ -148,9 +153,10 @@ _SIZES3 = [(70, 33, 20), (1, 1, 1), (2, 3, 2), (3, 1, 5), (63, 5, 4), (62, 2, 3)
 def test_batch3d_tile_shapes(variant, affinity, storage, monkeypatch):
     """Every 3D tile shape of the batched launch (columns / rows per lane, prefetch
     depth) on tiny, odd-width and multi-strip grids, short z segments, two step
-    counts (two groups), persistent grids of 1, 2 and all waves per SIMD, graph
-    replay; against the oracle."""
+    counts (two groups), persistent grids of 1, 2 and all waves per SIMD and
+    ordinary one-tile-per-wave grids, graph replay; against the oracle."""
     monkeypatch.setenv("SSAM_BATCH3D_WPS", str(variant % 3))
+    monkeypatch.setenv("SSAM_BATCH3D_GRID", str((variant // 2) % 2))
     monkeypatch.setenv("SSAM_BATCH3D_VARIANT", str(variant))
     monkeypatch.setenv("SSAM_BATCH3D_AFFINITY", str(affinity))
     monkeypatch.setenv("SSAM_BATCH3D_ALTERNATE", str(variant // 4))

@@ -35,10 +35,10 @@ def form(request, monkeypatch):
 
 @pytest.mark.parametrize("r", range(1, 10))
 @pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
-# pitches are multiples of 8 (the matrix-core form's row alignment) except the
-# last, which runs the VALU kernels under either setting
+# pitches that are multiples of 8 elements (16-byte rows) and odd ones (rows at
+# 2-byte offsets: unaligned 16-byte row loads)
 @pytest.mark.parametrize("h,w,pad", [(97, 413, 3), (16, 64, 8), (130, 250, 6), (1, 9, 7), (300, 1024, 0),
-                                     (50, 101, 0)])
+                                     (50, 101, 0), (33, 77, 2), (40, 130, 5)])
 def test_seeded_box(form, r, storage, h, w, pad):
     x = oracle.uniform(70 + r, h * w, -1, 1, np.float64).reshape(h, w)
     xs = torch.from_numpy(x).to(storage).to(torch.float64).numpy()  # the stored input
