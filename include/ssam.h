@@ -237,7 +237,8 @@ int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles
  * and the next ssam_batch_run captures a new graph. */
 int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
 /* The plan's tiling: 2D tile height per shape (4 entries: 5-point star, 3x3,
- * 5x5, 11x11 box), 3D tile-shape index and z-segment length, and 9 scheduling
+ * 5x5, 11x11 box), 3D tile-shape index (0..7; 8: per grid, the better of shapes
+ * 5 and 7 by width) and z-segment length, and 9 scheduling
  * entries: queue affinity of the 2D and of the 3D launches (1: waves start at
  * their own XCD's queue, 0: all waves work through the tiles in table order),
  * whether 3D sweeps alternate direction (odd steps walk the table back to front),

@@ -466,17 +466,33 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
         const int rc = get_scratch(nparts + 1, &scratch);
         if (rc != SSAM_OK) return rc;
     }
+    // launches: two sweeps each where the temporal form applies (radius 1, grid
+    // under 2 GiB, SSAM_3D_T != 1), the rest one sweep; a residual request runs
+    // the last sweep on its own (the single-sweep kernel forms the residual).
+    // Launches ping-pong so the last one writes dst.
+    std::vector<int> sweeps;
+    {
+        const bool t2 = steps >= 2 && two_sweeps_ok(storage, rq) && env_int("SSAM_3D_T", 2) != 1;
+        int left = steps;
+        const int tail = residual_l2 ? 1 : 0;
+        while (left > 0) {
+            const int n = (t2 && left - tail >= 2) ? 2 : 1;
+            sweeps.push_back(n);
+            left -= n;
+        }
+    }
+    const int nl = static_cast<int>(sweeps.size());
     const void* in = src;
-    for (int i = 0; i < steps; ++i) {
-        void* out = ((steps - 1 - i) % 2 == 0) ? dst : work;
+    for (int i = 0; i < nl; ++i) {
+        void* out = ((nl - 1 - i) % 2 == 0) ? dst : work;
         rq.src = in;
         rq.dst = out;
-        if (residual_l2 && i == steps - 1) {
+        if (residual_l2 && i == nl - 1) {
             rq.partials = scratch + 1;
             rq.partials_cap = nparts;
             rq.res_out = scratch;
         }
-        const int rc = dispatch3d(storage, rq, st);
+        const int rc = sweeps[i] == 2 ? dispatch3d_two_sweeps(storage, rq, st) : dispatch3d(storage, rq, st);
         if (rc != SSAM_OK) return rc;
         in = out;
     }

@@ -341,8 +341,6 @@ int build_group2d(Plan* P, Plan::Group& g) {
 template <typename S>
 int build_group3d(Plan* P, Plan::Group& g) {
     const int variant = P->var3d, zseg = P->zseg3d;
-    const int V = sizeof(S) == 8 ? 1 : kVar3D[variant].v, NY = kVar3D[variant].ny;
-    const int OW = (kWave - 2) * V;
     std::vector<BatchItem3D> h(g.ids.size());
     std::vector<S> taps;
     std::vector<ItemTiles> per(g.ids.size());
@@ -352,8 +350,12 @@ int build_group3d(Plan* P, Plan::Group& g) {
         b.src = it.src;
         b.dst = it.dst;
         b.work = it.work;
-        b.steps = it.steps;
         b.taps_off = static_cast<int>(taps.size());
+        b.var = variant == kVarPair3D ? pair_choice(it.nx, sizeof(S)) : variant;
+        b.steps = passes3d(static_cast<int>(it.steps), variant);
+        const int T = kVar3D[variant].t;
+        const int V = var3d_v(b.var, sizeof(S)), NY = kVar3D[b.var].ny;
+        const int OW = (kWave - 2 * (T == 2 && V == 1 ? 2 : 1)) * V;  // halo lanes: two columns per sweep
         const int nt = g.kind ? 19 : 7;
         for (int t = 0; t < nt; ++t) taps.push_back(static_cast<S>(it.coeffs[t]));
         Grid3D& gr = b.g;
@@ -375,8 +377,9 @@ int build_group3d(Plan* P, Plan::Group& g) {
                 for (int s = 0; s < b.nstrips; ++s) {
                     const int rows = std::min(NY, gr.ny - y * NY);
                     per[k].tiles.push_back(int4{static_cast<int>(k), s, y, z0 | (planes << 20)});
-                    // rows loaded x planes streamed (+2 halo planes) x taps
-                    per[k].cost.push_back(static_cast<double>(rows + 2) * (planes + 2) * nt);
+                    // rows loaded x planes streamed (+2 halo planes per sweep) x taps
+                    // x sweeps
+                    per[k].cost.push_back(static_cast<double>(rows + 2 * T) * (planes + 2 * T) * nt * T);
                 }
         }
     }
@@ -492,6 +495,9 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             if (rc != SSAM_OK) return rc;
         }
     }
+    // passes: one sweep each, or two (temporal variants; the last pass of an odd
+    // sweep count runs one)
+    const bool t2 = kVar3D[P->var3d].t == 2;
     int maxsteps = 0;
     for (auto& g : P->groups)
         if (g.ndim == 3) maxsteps = std::max(maxsteps, g.steps);
@@ -499,7 +505,9 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
         for (auto& g : P->groups) {
             if (g.ndim != 3) continue;
             int* cnt = counters + kCounterSet * ctr++;
-            if (!(parts & kRun3D) || s >= g.steps) continue;
+            const int npass = passes3d(g.steps, P->var3d);
+            if (!(parts & kRun3D) || s >= npass) continue;
+            int single = t2 && (g.steps % 2 == 1) && s == npass - 1;
             const int blocks = std::max(1, std::min(g.blocks, (g.ntiles + kBatchWavesPerBlock - 1) /
                                                                   kBatchWavesPerBlock));
             const BatchItem3D* items = static_cast<const BatchItem3D*>(g.d_items);
@@ -507,7 +515,7 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             QueueSet qs = g.qs;
             int step = s;
             const void* taps = g.d_taps;
-            void* args[] = {&items, &tiles, &qs, &cnt, &taps, &step};
+            void* args[] = {&items, &tiles, &qs, &cnt, &taps, &step, &single};
             const void* fn = batch3d_kernel_fn(g.storage, P->var3d, g.kind, P->grid3d != 0);
             if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
                     hipSuccess ||
@@ -746,7 +754,8 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
     if (has3) {
         // tile shape x z-segment length x launch form (persistent / grid), then
         // the persistent grid's occupancy, queue affinity and step direction
-        search({&P->var3d, &P->zseg3d, &P->grid3d}, {{0, 1, 2, 3, 4, 5, 6, 7}, {16, 32, 64}, {0, 1}}, kRun3D);
+        search({&P->var3d, &P->zseg3d, &P->grid3d},
+               {{0, 1, 2, 3, 4, 5, 6, 7, kVarPair3D, 9, 10, 11, 12}, {16, 32, 64}, {0, 1}}, kRun3D);
         if (!P->grid3d) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
         search({&P->aff3d, &P->alt3d}, {{0, 1}, {0, 1}}, kRun3D);
     }

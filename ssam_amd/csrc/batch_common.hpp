@@ -55,8 +55,9 @@ struct BatchItem3D {
     Grid3D g;
     int nstrips, nyb;
     int vec;       // nx % V == 0 and every buffer aligned to V elements
-    int steps;
+    int steps;     // passes over the grid (sweeps / sweeps per pass, rounded up)
     int taps_off;
+    int var;       // tile shape of this item's tiles (a kVar3D index)
 };
 
 constexpr int kBatchWavesPerBlock = 4;
@@ -146,10 +147,18 @@ template <typename S> constexpr bool kHalf2D = std::is_same<S, bf16_t>::value ||
 // 64..256-column row); more rows per lane cut the y-halo re-reads.
 // wpe: waves per SIMD the register allocation must leave room for (bounds the
 // compiler's load hoisting to what each shape needs).
-struct Var3D { int v, ny, pf, wpe; };
-constexpr Var3D kVar3D[] = {{1, 8, 2, 4}, {2, 16, 1, 2}, {2, 4, 2, 4}, {2, 8, 1, 3},
-                            {2, 8, 2, 2}, {2, 12, 2, 2}, {1, 16, 2, 2}, {4, 8, 1, 2}};
-constexpr int kNumVar3D = 8;
+// t: sweeps per pass (2: temporal blocking, Stencil3DHL2).
+struct Var3D { int v, ny, pf, wpe, t; };
+// Variant index kVarPair3D: per item, the better of shapes kPairA (narrow
+// grids) and kPairB (wider ones), in one kernel (pair_choice below); its table
+// entry repeats kPairA's shape.
+constexpr int kVarPair3D = 8;
+constexpr int kPairA = 5, kPairB = 7;
+constexpr Var3D kVar3D[] = {{1, 8, 2, 4, 1},  {2, 16, 1, 2, 1}, {2, 4, 2, 4, 1}, {2, 8, 1, 3, 1},
+                            {2, 8, 2, 2, 1},  {2, 12, 2, 2, 1}, {1, 16, 2, 2, 1}, {4, 8, 1, 2, 1},
+                            {2, 12, 2, 2, 1}, {2, 8, 1, 2, 2},  {2, 6, 1, 3, 2},  {4, 4, 1, 2, 2},
+                            {1, 16, 1, 2, 2}};
+constexpr int kNumVar3D = 13;
 constexpr int kDefaultVar3D = 4;
 constexpr int kDefaultZseg3D = 32;
 constexpr int kDefaultSeg2D = 64;
@@ -158,8 +167,32 @@ template <typename S, int VAR> struct Var3DT {
     // FP64 runs one column per lane (a plane row of NY+2 doubles per lane)
     static constexpr int V = sizeof(S) == 8 ? 1 : kVar3D[VAR].v;
     static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
-    static constexpr int WPE = sizeof(S) == 8 ? (NY >= 16 ? 1 : 2) : kVar3D[VAR].wpe;
+    static constexpr int WPE = sizeof(S) == 8 ? (NY >= 16 || kVar3D[VAR].t == 2 ? 1 : 2) : kVar3D[VAR].wpe;
+    static constexpr int T = kVar3D[VAR].t;
 };
+
+// Passes of a 3D item of `steps` sweeps with a variant of t sweeps per pass (the
+// last pass of an odd count runs one sweep).
+inline int passes3d(int steps, int variant) { return kVar3D[variant].t == 2 ? (steps + 1) / 2 : steps; }
+
+// Columns per lane of a variant for storage of `elem_bytes` bytes.
+inline int var3d_v(int variant, int elem_bytes) { return elem_bytes == 8 ? 1 : kVar3D[variant].v; }
+
+// The pair's shape for a grid nx columns wide: the one with the least estimated
+// time per point, lane slots (62*V-column strips, the last one partly empty) x
+// y-halo rows ((NY+2)/NY) / the measured rate of a full lane slot of that
+// shape (wider V issues fewer instructions per byte: 512, 630 and 797 Gpoints/s
+// per lane-slot share for V = 1, 2, 4 on one MI355X).
+inline int pair_choice(long long nx, int elem_bytes) {
+    auto cost = [&](int var) {
+        const int V = var3d_v(var, elem_bytes), NY = kVar3D[var].ny;
+        const double rate = V == 1 ? 512.0 : (V == 2 ? 630.0 : 797.0);
+        const long long ow = (kWave - 2) * V;
+        const double slots = static_cast<double>((nx + ow - 1) / ow * ow);
+        return slots * (NY + 2) / NY / rate;
+    };
+    return cost(kPairB) < cost(kPairA) ? kPairB : kPairA;
+}
 
 // Kernel entry points by launch parameters (host stubs for hipLaunchKernel and
 // the occupancy query).  storage: SSAM_F32 / F64 / BF16 / F16; kind: a Kind2D

@@ -48,6 +48,9 @@ struct Req3D {
 };
 
 int dispatch3d(int storage, const Req3D& rq, hipStream_t st);
+// Two sweeps in one launch (radius 1, grid under 2 GiB); SSAM_E_UNSUPPORTED else.
+bool two_sweeps_ok(int storage, const Req3D& rq);
+int dispatch3d_two_sweeps(int storage, const Req3D& rq, hipStream_t st);
 int partials_needed(int storage, const Req3D& rq);
 
 }  // namespace ssam

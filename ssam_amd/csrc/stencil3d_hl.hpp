@@ -26,6 +26,8 @@
 
 namespace ssam {
 
+constexpr int cgcd3(int a, int b) { return b == 0 ? a : cgcd3(b, a % b); }
+
 struct Grid3D {
     int nx, ny, nz;
     long long sy, sz;  // element strides of y and z (dense: nx, nx*ny)
@@ -183,6 +185,220 @@ struct Stencil3DHL {
         const int x0 = strip * OW + (lane - 1) * V;
         c.lo = static_cast<uint32_t>(x0) * ES;  // x0 = -V wraps: reads past the end
         const bool store_lane = lane >= 1 && lane < kWave - 1;
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            const int x = x0 + j;
+            c.so[j] = (store_lane && x < g.nx) ? c.lo + static_cast<uint32_t>(j) * ES : kBufOOB;
+            c.xin[j] = x >= 1 && x < g.nx - 1;
+        }
+        if (V == 1 || vec) stream<true>(c, w, zs, ze);
+        else stream<false>(c, w, zs, ze);
+    }
+};
+
+constexpr int clcm(int a, int b) { return a / cgcd3(a, b) * b; }
+
+// Two sweeps per pass (temporal blocking, the reference's IT = 2): the wave
+// streams z once and keeps two register rings, input planes (NY+4 rows: the
+// second sweep's halo needs two rows above and below) and the first sweep's
+// planes (NY+2 rows, three planes).  Tick q loads input plane q+2+PF, computes
+// first-sweep plane q+1 from input planes q..q+2 and, once q >= zs, output
+// plane q from first-sweep planes q-1..q+1.  First-sweep values outside the
+// grid's interior are 0, exactly what the first sweep's output holds there
+// (zero Dirichlet), so two passes of this body equal four single sweeps bit for
+// bit.  Columns: 2*HL halo lanes (one lane of V >= 2 columns, two of V = 1).
+template <typename S, int SHAPE, int V, int NY, int PF>
+struct Stencil3DHL2 {
+    using A = typename Storage<S>::acc;
+    static_assert(sizeof(S) == sizeof(A), "F32 / F64 storage");
+    static_assert(PF >= 1, "prefetch at least one plane");
+    static constexpr int R = 1;
+    static constexpr int NT = ntaps3(SHAPE, R);
+    static constexpr int HL = V >= 2 ? 1 : 2;
+    static constexpr int OW = (kWave - 2 * HL) * V;
+    static constexpr int NI = NY + 4;            // input rows y0-2 .. y0+NY+1
+    static constexpr int NM = NY + 2;            // first-sweep rows y0-1 .. y0+NY
+    static constexpr int NPI = 3 + PF;           // input planes in the ring
+    static constexpr int PERIOD = clcm(NPI, 3);  // ticks until both rings realign
+    static constexpr uint32_t ES = sizeof(S);
+
+    struct PlaneI { A v[NI][V]; };
+    struct PlaneM { A v[NM][V]; };
+
+    struct Ctx {
+        BufRsrc src, dst;
+        int ny, nz, zs;
+        uint32_t sy, sz;
+        int y0;
+        int ze;
+        bool single;          // one sweep only (first-sweep planes are the output)
+        uint32_t lo;
+        uint32_t so[V];
+        bool xin[V];
+    };
+
+    template <bool VEC>
+    __device__ static __forceinline__ void store_row(const Ctx& c, uint32_t roff, const A (&out)[V]) {
+        if constexpr (VEC) {
+            buf_store<A, V>(c.dst, c.so[0] + roff, out);
+        } else {
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const A e[1] = {out[j]};
+                buf_store<A, 1>(c.dst, c.so[j] + roff, e);
+            }
+        }
+    }
+
+    template <bool VEC>
+    __device__ static __forceinline__ void load_plane(const Ctx& c, int z, PlaneI& P) {
+        const uint32_t pz = static_cast<uint32_t>(z) * c.sz + static_cast<uint32_t>(c.y0 - 2) * c.sy;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const uint32_t off = c.lo + pz + static_cast<uint32_t>(i) * c.sy;
+            if constexpr (VEC) {
+                buf_load<A, V>(c.src, off, P.v[i]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    A e[1];
+                    buf_load<A, 1>(c.src, off + static_cast<uint32_t>(j) * ES, e);
+                    P.v[i][j] = e[0];
+                }
+            }
+        }
+    }
+
+    template <int NR>
+    __device__ static __forceinline__ A at(const A (&v)[NR][V], int i, int j, int dx) {
+        const int cc = j + dx;
+        if (cc >= 0 && cc < V) return v[i][cc];
+        if (cc < 0) return from_left(v[i][V - 1]);
+        return from_right(v[i][0]);
+    }
+
+    // tick KT of a period starting at q = zq0 (zq0 - (zs - 2) is a multiple of
+    // PERIOD): input plane zq0+k sits in slot k % NPI, first-sweep plane zq0+k in
+    // slot k % 3.
+    template <bool VEC, int KT>
+    __device__ static __forceinline__ void tick(const Ctx& c, const A* __restrict__ w, int zq0,
+                                                PlaneI (&in)[NPI], PlaneM (&mid)[3]) {
+        const int q = zq0 + KT;
+        load_plane<VEC>(c, q + 2 + PF, in[(KT + 2 + PF) % NPI]);
+        int zero = 0;
+        asm volatile("" : "+s"(zero));
+        const A* __restrict__ tw = w + zero;
+        // first sweep: plane m = q+1, rows y0-1 .. y0+NY
+        {
+            const int m = q + 1;
+            const bool zin = m >= 1 && m < c.nz - 1;
+            PlaneM& M = mid[(KT + 1) % 3];
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                const int y = c.y0 - 1 + i;
+                const bool yin = zin && y >= 1 && y < c.ny - 1;
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    A acc = 0;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const Tap3 tp = tap3(SHAPE, t);
+                        const PlaneI& P = in[(KT + 1 + tp.dz) % NPI];
+                        acc = fmaa(tw[t], at(P.v, i + 1 + tp.dy, j, tp.dx), acc);
+                    }
+                    M.v[i][j] = (yin && c.xin[j]) ? acc : A(0);
+                }
+            }
+            if (c.single && m >= c.zs && m < c.ze) {
+                const uint32_t mz = static_cast<uint32_t>(m) * c.sz;
+#pragma unroll
+                for (int i = 1; i <= NY; ++i) {
+                    const int y = c.y0 - 1 + i;
+                    if (y < c.ny) store_row<VEC>(c, mz + static_cast<uint32_t>(y) * c.sy, M.v[i]);
+                }
+            }
+        }
+        // second sweep: output plane q, rows y0 .. y0+NY-1
+        if (!c.single && q >= c.zs) {
+            const bool zin = q >= 1 && q < c.nz - 1;
+            const uint32_t qz = static_cast<uint32_t>(q) * c.sz;
+#pragma unroll
+            for (int i = 0; i < NY; ++i) {
+                const int y = c.y0 + i;
+                const bool yin = zin && y >= 1 && y < c.ny - 1;
+                A out[V];
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    A acc = 0;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const Tap3 tp = tap3(SHAPE, t);
+                        const PlaneM& M = mid[(KT + 3 + tp.dz) % 3];
+                        acc = fmaa(tw[t], at(M.v, i + 1 + tp.dy, j, tp.dx), acc);
+                    }
+                    out[j] = (yin && c.xin[j]) ? acc : A(0);
+                }
+                if (y < c.ny) store_row<VEC>(c, qz + static_cast<uint32_t>(y) * c.sy, out);
+            }
+        }
+    }
+
+    template <bool VEC, int KT>
+    __device__ static __forceinline__ void period(const Ctx& c, const A* __restrict__ w, int zq0,
+                                                  PlaneI (&in)[NPI], PlaneM (&mid)[3]) {
+        if constexpr (KT < PERIOD) {
+            tick<VEC, KT>(c, w, zq0, in, mid);
+            period<VEC, KT + 1>(c, w, zq0, in, mid);
+        }
+    }
+
+    template <bool VEC, int KT>
+    __device__ static __forceinline__ void tail(const Ctx& c, const A* __restrict__ w, int zq0, int zend,
+                                                PlaneI (&in)[NPI], PlaneM (&mid)[3]) {
+        if constexpr (KT < PERIOD - 1) {
+            if (zq0 + KT < zend) {
+                tick<VEC, KT>(c, w, zq0, in, mid);
+                tail<VEC, KT + 1>(c, w, zq0, zend, in, mid);
+            }
+        }
+    }
+
+    template <bool VEC>
+    __device__ static __forceinline__ void stream(const Ctx& c, const A* __restrict__ w, int zs, int ze) {
+        PlaneI in[NPI];
+        PlaneM mid[3];
+        // ticks run q = zs-2 .. ze-1; the first tick needs input planes
+        // zs-2 .. zs+PF-1 in slots 0 .. PF+1
+#pragma unroll
+        for (int k = 0; k < 2 + PF; ++k) load_plane<VEC>(c, zs - 2 + k, in[k]);
+        int zq0 = zs - 2;
+#pragma nounroll
+        for (; zq0 + PERIOD <= ze; zq0 += PERIOD) period<VEC, 0>(c, w, zq0, in, mid);
+        tail<VEC, 0>(c, w, zq0, ze, in, mid);
+    }
+
+    // One tile: strip `strip`, y-block `yb`, output planes [zs, ze) of a dense
+    // grid that fits grid3d_fits_buffer, two sweeps from src to dst (single: one).
+    __device__ static __forceinline__ void run_tile(const S* __restrict__ src, S* __restrict__ dst,
+                                                    const Grid3D& g, bool vec, const A* __restrict__ w,
+                                                    int strip, int yb, int zs, int ze, bool single) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const uint32_t bytes = static_cast<uint32_t>(g.nx) * static_cast<uint32_t>(g.ny) *
+                               static_cast<uint32_t>(g.nz) * ES;
+        Ctx c;
+        c.src = buf_rsrc(src, bytes);
+        c.dst = buf_rsrc(dst, bytes);
+        c.ny = g.ny;
+        c.nz = g.nz;
+        c.zs = zs;
+        c.ze = ze;
+        c.single = single;
+        c.sy = static_cast<uint32_t>(g.sy) * ES;
+        c.sz = static_cast<uint32_t>(g.sz) * ES;
+        c.y0 = yb * NY;
+        const int x0 = strip * OW + (lane - HL) * V;
+        c.lo = static_cast<uint32_t>(x0) * ES;
+        const bool store_lane = lane >= HL && lane < kWave - HL;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
             const int x = x0 + j;
