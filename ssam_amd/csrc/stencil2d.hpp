@@ -261,9 +261,12 @@ struct Stencil2D {
         }
     }
 
+    // BF16 / F16: cv holds the window rows converted to FP32, each row converted
+    // once, at the tick where it enters the window (the packed ring slot keeps the
+    // load's result until then); unused for F32 / F64.
     template <bool CHK, int KT>
     __device__ static __forceinline__ void tick(const Lane& L, const A* __restrict__ w, int yc0,
-                                                Row (&ring0)[NR0],
+                                                Row (&ring0)[NR0], A (&cv)[NR0][V],
                                                 A (&rings)[NSTAGE_RINGS][NRS][V]) {
         const int yc = yc0 + KT;
         // rows past the tile's last needed row only feed discarded ticks: re-read
@@ -275,6 +278,10 @@ struct Stencil2D {
         int zero = 0;
         asm volatile("" : "+s"(zero));
         const A* __restrict__ tw = w + zero;
+        if constexpr (Row::kPacked) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) cv[s0(KT, 2 * R)][j] = ring0[s0(KT, 2 * R)].get(j);
+        }
 #pragma unroll
         for (int t = 1; t <= T; ++t) {
             A X[K][V];
@@ -282,7 +289,8 @@ struct Stencil2D {
             for (int n = 0; n < K; ++n)
 #pragma unroll
                 for (int j = 0; j < V; ++j)
-                    X[n][j] = (t == 1) ? ring0[s0(KT, n)].get(j) : rings[t - 2][sS(KT, n)][j];
+                    X[n][j] = (t == 1) ? (Row::kPacked ? cv[s0(KT, n)][j] : ring0[s0(KT, n)].get(j))
+                                       : rings[t - 2][sS(KT, n)][j];
             A out[V];
             stencil_row<A, R, SHAPE, V>(X, tw, out);
             const int q = yc - t * R;  // grid row produced by this step
@@ -320,11 +328,11 @@ struct Stencil2D {
 
     template <bool CHK, int KT>
     __device__ static __forceinline__ void block(const Lane& L, const A* __restrict__ w, int yc0,
-                                                 Row (&ring0)[NR0],
+                                                 Row (&ring0)[NR0], A (&cv)[NR0][V],
                                                  A (&rings)[NSTAGE_RINGS][NRS][V]) {
         if constexpr (KT < U) {
-            tick<CHK, KT>(L, w, yc0, ring0, rings);
-            block<CHK, KT + 1>(L, w, yc0, ring0, rings);
+            tick<CHK, KT>(L, w, yc0, ring0, cv, rings);
+            block<CHK, KT + 1>(L, w, yc0, ring0, cv, rings);
         }
     }
 
@@ -332,6 +340,7 @@ struct Stencil2D {
     __device__ static __forceinline__ void stream(const Lane& L, const A* __restrict__ w, int c0,
                                                   int nblk) {
         Row ring0[NR0];
+        A cv[NR0][V];
         A rings[NSTAGE_RINGS][NRS][V];
 #pragma unroll
         for (int s = 0; s < NSTAGE_RINGS; ++s)
@@ -344,6 +353,12 @@ struct Stencil2D {
         // rows c0-2R .. c0+PF-1 into slots 0 .. 2R+PF-1
 #pragma unroll
         for (int s = 0; s < 2 * R + PF; ++s) load_row<CHK>(L, min(c0 - 2 * R + s, L.ylast), ring0[s]);
+        if constexpr (Row::kPacked) {
+#pragma unroll
+            for (int s = 0; s < 2 * R; ++s)
+#pragma unroll
+                for (int j = 0; j < V; ++j) cv[s][j] = ring0[s].get(j);
+        }
         int yc0 = c0;
         for (int b = 0; b < nblk; ++b, yc0 += U) {
             if constexpr (!RING) {
@@ -351,10 +366,16 @@ struct Stencil2D {
                 for (int k = 0; k < U; ++k)
                     load_row<CHK>(L, min(yc0 + k + PF, L.ylast), ring0[2 * R + PF + k]);
             }
-            block<CHK, 0>(L, w, yc0, ring0, rings);
+            block<CHK, 0>(L, w, yc0, ring0, cv, rings);
             if constexpr (!RING) {
 #pragma unroll
                 for (int i = 0; i < 2 * R + PF; ++i) ring0[i] = ring0[i + U];
+                if constexpr (Row::kPacked) {
+#pragma unroll
+                    for (int i = 0; i < 2 * R; ++i)
+#pragma unroll
+                        for (int j = 0; j < V; ++j) cv[i][j] = cv[i + U][j];
+                }
 #pragma unroll
                 for (int s = 0; s < NSTAGE_RINGS; ++s)
 #pragma unroll

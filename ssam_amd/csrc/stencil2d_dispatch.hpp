@@ -20,8 +20,8 @@ template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2
 
 
 // Loop form per instance.  RING (no register moves): the body is a full ring
-// period, lcm(2r+1+PF, 2r+1) ticks, used while that stays under about 4096 FMAs
-// of code.  Otherwise blocks of U ticks with a shift (U bounded by about 1024
+// period, lcm(2r+1+PF, 2r+1) ticks, used while that stays under about 2048 FMAs
+// of code.  Otherwise blocks of U ticks with a shift (U bounded by about 2048
 // FMAs per block).  Also the minimum waves per SIMD the register allocator must
 // leave room for.
 template <int R, int SHAPE, int V, int T> struct Unroll2D {
