@@ -290,6 +290,7 @@ def main():
     extras = {}
     if not a.no_extras:
         extras = side_runs(suite, taps, stream, per, tune=not a.no_tune)
+        extras["config0_star5_hold"] = side_config0(stream, copy_gbps)
         extras["config2_3d"] = side_3d(stream, copy_gbps)
         extras["config3_conv"] = side_conv(stream, copy_gbps, tune=not a.no_tune)
         extras["config4_batch"] = side_batch(stream)
@@ -393,6 +394,29 @@ def _time(fn, stream, reps=3):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def side_config0(stream, copy_gbps):
+    """Config 0: one sweep of the 5-point star (centre 0.5, N/S/E/W 0.125) on 2048^2
+    FP32, values U[0,1) from seed 0, the 1-cell border held fixed (HOLD).  Parity
+    with the gold at this exact size is a GPU test; the 32 MiB working set stays in
+    the last-level cache across repetitions, so the rate is not an HBM figure."""
+    import numpy as np
+    import torch
+
+    from ssam_amd import _lib as L
+    from ssam_amd import api
+
+    n = 2048
+    x = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    api.fill_uniform(x, 0, 0.0, 1.0)
+    y = torch.empty_like(x)
+    f = np.array([[0.0, 0.125, 0.0], [0.125, 0.5, 0.125], [0.0, 0.125, 0.0]])
+    api.tune2d(x, y, f, shape=L.STAR, boundary=L.HOLD, stream=stream)
+    ms = _time(lambda: api.stencil2d(x, y, f, shape=L.STAR, boundary=L.HOLD, stream=stream), stream, reps=20)
+    gbps = 8 * n * n / ms / 1e6
+    return dict(ms=round(ms, 4), gpts=round(n * n / ms / 1e6, 2), gbps=round(gbps, 1),
+                stream_frac=round(gbps / copy_gbps, 3))
 
 
 def side_3d(stream, copy_gbps):

@@ -239,3 +239,24 @@ def test_full_size_8192_bands(shape, r, steps):
     got = dst.cpu().numpy()
     assert np.isfinite(got).all()
     _band_check(x, got, f, [0, 2040, 4096 - 7, 8192 - 16], dt, steps)
+
+
+def test_config0_star5_held_border():
+    """The plain 5-point workload at its stated size: one sweep on 2048^2 FP32,
+    values U[0,1) from seed 0 (the device fill and the oracle draw the same
+    splitmix64 stream), centre 0.5 and N/S/E/W 0.125, the 1-cell border held fixed;
+    every point against the gold."""
+    n = 2048
+    src = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    api.fill_uniform(src, 0, 0.0, 1.0)
+    x = oracle.uniform(0, n * n, 0, 1, np.float32).reshape(n, n)
+    assert np.array_equal(src.cpu().numpy(), x)
+    f = np.array([[0.0, 0.125, 0.0], [0.125, 0.5, 0.125], [0.0, 0.125, 0.0]])
+    dst = torch.full_like(src, float("nan"))
+    api.stencil2d(src, dst, f, shape=L.STAR, boundary=L.HOLD)
+    torch.cuda.synchronize()
+    got = dst.cpu().numpy()
+    want = oracle.conv2d(x, f.astype(np.float32))
+    want[0, :], want[-1, :], want[:, 0], want[:, -1] = x[0, :], x[-1, :], x[:, 0], x[:, -1]
+    assert np.array_equal(got[0, :], x[0, :]) and np.array_equal(got[:, -1], x[:, -1])
+    assert_close(got, want, np.maximum(scale_of(x, f), np.abs(x)), TOL[np.float32], "config 0")
