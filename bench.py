@@ -55,7 +55,32 @@ def parse():
     ap.add_argument("--ingest-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="no GPU: turn two rocprofv3 --pmc runs of this bench (FETCH_SIZE, then "
                          "WRITE_SIZE) into per-kernel HBM bytes under profiles/ and exit")
+    ap.add_argument("--side", choices=("config0", "config2", "config3", "config4"),
+                    help="run only this side measurement and print its JSON (for profiling)")
+    ap.add_argument("--ingest-stats", nargs=3, metavar=("STATS_DIR", "NAME", "COMMAND"),
+                    help="no GPU: summarise a rocprofv3 --kernel-trace --stats run (its "
+                         "kernel_stats.csv) into profiles/NAME.md, quoting COMMAND, and exit")
     return ap.parse_args()
+
+
+def ingest_stats(stats_dir: str, name: str, command: str) -> None:
+    """Per-kernel calls, average and total duration of a rocprofv3 --stats run."""
+    import csv
+
+    f = next(pathlib.Path(stats_dir).rglob("*kernel_stats.csv"))
+    rows = list(csv.DictReader(open(f)))
+    total = sum(float(r["TotalDurationNs"]) for r in rows) or 1.0
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = [f"# {name}", "", "Command (repository root, on one MI355X):", "", "```", command, "```", "",
+           "| kernel | calls | avg us | min us | max us | share of GPU time |",
+           "|---|---|---|---|---|---|"]
+    for r in rows[:24]:
+        out.append(f"| `{r['Name'][:150]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                   f"{100 * float(r['TotalDurationNs']) / total:.1f}% |")
+    dst = pathlib.Path(__file__).resolve().parent / "profiles" / f"{name}.md"
+    dst.write_text("\n".join(out) + "\n")
+    print(dst)
 
 
 def ingest_trace(trace_dir: str, steps: int, warmup: int) -> None:
@@ -174,6 +199,9 @@ def main():
     if a.ingest_trace:
         ingest_trace(a.ingest_trace, a.steps, a.warmup)
         return
+    if a.ingest_stats:
+        ingest_stats(*a.ingest_stats)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SSAM stencil path runs on a single MI355X here "
               "(multi-GPU slab decomposition is described in DESIGN.md, not run)", file=sys.stderr)
@@ -190,6 +218,14 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
+    if a.side:
+        copy_gbps = 5200.0  # nominal for the fractions; the full bench measures it
+        fn = {"config0": lambda: side_config0(stream, copy_gbps),
+              "config2": lambda: side_3d(stream, copy_gbps),
+              "config3": lambda: side_conv(stream, copy_gbps, tune=not a.no_tune),
+              "config4": lambda: side_batch(stream)}[a.side]
+        print(json.dumps({a.side: fn()}))
+        return
     suite = [s for s in SUITE if not a.only or s[0] in a.only.split(",")]
 
     def taps(shape, r):

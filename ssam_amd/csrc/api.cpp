@@ -590,7 +590,7 @@ int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst
                            ? kStar
                            : ((d->flags & SSAM_FLAG_COMPENSATED) ? kBoxKahan : kBox);
     const int fmax = max_fused(d->storage, d->shape, R);
-    const int wps_c[] = {2, 4, 8, 12};
+    const int wps_c[] = {2, 4, 8, 12, 24};
     std::vector<int> tch_c;
     for (int t = fmax; t >= 1; t >>= 1) {
         if (d->steps > 1 && t > d->steps) continue;

@@ -69,7 +69,8 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
         long long nsegs = (target_waves + p.nstrips - 1) / p.nstrips;
         if (nsegs < 1) nsegs = 1;
         seg = static_cast<int>((rq.height + nsegs - 1) / nsegs);
-        const int min_seg = 8 * T * R > 16 ? 8 * T * R : 16;
+        // (8 rows at least: small grids need many short tiles to fill the device)
+        const int min_seg = 8 * T * R > 8 ? 8 * T * R : 8;
         if (seg < min_seg) seg = min_seg;
     }
     if (seg > p.height) seg = p.height;
