@@ -396,7 +396,10 @@ const double kJacobi19[19] = {0.0,        1.0 / 12,   1.0 / 12,   1.0 / 12,   1.
                               1.0 / 24,   1.0 / 24,   1.0 / 24,   1.0 / 24};
 
 // Device scratch for residual partials: grown on demand, kept per device for the
-// life of the process (never a grid-sized buffer).
+// life of the process (never a grid-sized buffer).  A residual call holds `mu`
+// from taking the scratch until its stream has finished with it (the call
+// synchronises to return the norm), so concurrent calls on other host threads
+// never share or free it under each other.
 struct Scratch {
     std::mutex mu;
     double* ptr[64] = {};
@@ -404,10 +407,10 @@ struct Scratch {
 };
 Scratch g_scratch;
 
+// The caller holds g_scratch.mu.
 int get_scratch(int nwords, double** out) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return SSAM_E_HIP;
-    std::lock_guard<std::mutex> lk(g_scratch.mu);
     if (g_scratch.cap[dev] < nwords) {
         if (g_scratch.ptr[dev]) (void)hipFree(g_scratch.ptr[dev]);
         g_scratch.ptr[dev] = nullptr;
@@ -461,7 +464,9 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
     rq.res_scale = res_scale;
     double* scratch = nullptr;
     int nparts = 0;
+    std::unique_lock<std::mutex> scratch_lock;
     if (residual_l2) {
+        scratch_lock = std::unique_lock<std::mutex>(g_scratch.mu);
         nparts = partials_needed(storage, rq);
         const int rc = get_scratch(nparts + 1, &scratch);
         if (rc != SSAM_OK) return rc;
@@ -493,7 +498,11 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
             rq.res_out = scratch;
         }
         const int rc = sweeps[i] == 2 ? dispatch3d_two_sweeps(storage, rq, st) : dispatch3d(storage, rq, st);
-        if (rc != SSAM_OK) return rc;
+        if (rc != SSAM_OK) {
+            // launches already queued may use the scratch: let them finish first
+            if (residual_l2) (void)hipStreamSynchronize(st);
+            return rc;
+        }
         in = out;
     }
     if (residual_l2) {

@@ -135,3 +135,29 @@ def test_two_sweeps_per_launch_bit_exact(shape, dt, dims, steps, monkeypatch):
     assert np.array_equal(outs[0], outs[1])
     want = _gold(x, taps, coef, 1, steps)
     _check(outs[0], want, _scale(x, taps, coef, 1, steps), TOL[dt] * steps, f"{shape} {dims} x{steps}")
+
+
+def test_jacobi19_residual_concurrent_threads():
+    """Residual requests from several host threads, each on its own stream and
+    grid size (the partials scratch grows between them): every call returns the
+    norm a call alone returns."""
+    import concurrent.futures as cf
+
+    sizes = [24, 40, 56, 72]
+    grids = []
+    for k, n in enumerate(sizes):
+        x = oracle.uniform(300 + k, n ** 3, 0, 1, np.float64).reshape(n, n, n)
+        src = torch.from_numpy(x).cuda()
+        grids.append((src, torch.empty_like(src), torch.empty_like(src)))
+    alone = [api.jacobi19(s, d, 3, w) for s, d, w in grids]
+
+    def run(k):
+        s, d, w = grids[k]
+        st = torch.cuda.Stream()
+        with torch.cuda.stream(st):
+            return [api.jacobi19(s, d, 3, w, stream=st) for _ in range(4)]
+
+    with cf.ThreadPoolExecutor(max_workers=len(sizes)) as ex:
+        got = list(ex.map(run, range(len(sizes))))
+    for k in range(len(sizes)):
+        assert got[k] == [alone[k]] * 4, (sizes[k], got[k], alone[k])
