@@ -122,7 +122,15 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
         if (mfma_eligible<S>(rq) &&
             (rq.form == kFormMfma || (rq.form == kFormAuto && rq.radius >= 2)))
             return launch_mfma<S>(rq, st);
-        if (rq.shape == kStar) return dispatch_radius<S, kStar, V, 1>(rq, st);
+        if (rq.shape == kStar) {
+            // BF16/FP16 stars of radius >= 4: 4 columns per lane (8-byte rows per
+            // lane) keep the ring in the loop-period form with room for more
+            // waves; measured 13-36% faster than 8 columns at radius 5 and 6
+            if constexpr (sizeof(S) == 2) {
+                if (rq.radius >= 4) return dispatch_radius<S, kStar, VT1, 1>(rq, st);
+            }
+            return dispatch_radius<S, kStar, V, 1>(rq, st);
+        }
         // FP64 storage accumulates in FP64: compensation changes nothing there
         if (rq.shape == kBoxKahan && sizeof(S) != 8) return dispatch_radius<S, kBoxKahan, V, 1>(rq, st);
         return dispatch_radius<S, kBox, V, 1>(rq, st);

@@ -199,7 +199,8 @@ def test_box_multistep_chains_single_sweeps():
 
 @pytest.mark.parametrize("storage,tol", [(torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
 @pytest.mark.parametrize("shape,r,steps", [(L.STAR, 1, 1), (L.BOX, 2, 1), (L.BOX, 5, 1),
-                                           (L.STAR, 1, 8), (L.STAR, 2, 8), (L.STAR, 4, 1)])
+                                           (L.STAR, 1, 8), (L.STAR, 2, 8), (L.STAR, 4, 1), (L.STAR, 5, 1),
+                                           (L.STAR, 6, 1)])
 def test_storage_precision_vs_fp64(storage, tol, shape, r, steps):
     f = wl.normalised_taps("star" if shape == L.STAR else "box", r, 77)
     h, w = 150, 700
