@@ -195,15 +195,17 @@ int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t n
 /* ---------------------------------------------------------------- batches
  * Many independent grids of uneven sizes and mixed shapes.  2D items of one
  * storage type whose shape is a 5-point star or a 3x3 / 5x5 / 11x11 box run as
- * one persistent launch per stencil kind, back to back (or, as the tuner may
- * choose, ONE persistent launch for all kinds); 3D items (7-point star or 19-point face+edge, F32/F64)
- * of under 2 GiB run one persistent launch per sweep for each (storage, steps,
- * shape); any other item gets its own launch.  Persistent launches take tiles from 8 device work
- * queues, one per XCD (neighbouring tiles of a grid share that XCD's L2), and
- * steal from the other queues when their own is empty.  The first ssam_batch_run
- * captures the whole sequence into a hipGraph on a private stream and every run
- * replays it on `stream` (SSAM_BATCH_EAGER: launch directly).  Device pointers
- * are bound at plan creation; items and taps are copied.  No counterpart in the
+ * one launch per stencil kind (or one for all kinds); 3D items (7-point star or
+ * 19-point face+edge, F32/F64, under 2 GiB) run one launch per pass of one or two
+ * sweeps for each (storage, steps, shape); any other item gets its own launch.
+ * Each launch is an ordinary grid with one tile per wave over a table sorted by
+ * decreasing tile cost (default), or a persistent grid taking tiles from 8 device
+ * work queues, one per XCD (neighbouring tiles of a grid share that XCD's L2),
+ * stealing from the other queues when its own is empty; the tuner picks.  The
+ * first ssam_batch_run captures the whole sequence into a hipGraph on a private
+ * stream and every run replays it on `stream` (SSAM_BATCH_EAGER: launch
+ * directly).  Device pointers are bound at plan creation; items and taps are
+ * copied.  No counterpart in the
  * reference. */
 typedef struct {
     int32_t ndim;          /* 2 or 3 */
@@ -237,8 +239,9 @@ int ssam_batch_plan_info(ssam_batch_plan plan, int64_t* tiles_2d, int64_t* tiles
  * and the next ssam_batch_run captures a new graph. */
 int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
 /* The plan's tiling: 2D tile height per shape (4 entries: 5-point star, 3x3,
- * 5x5, 11x11 box), 3D tile-shape index (0..7; 8: per grid, the better of shapes
- * 5 and 7 by width) and z-segment length, and 9 scheduling
+ * 5x5, 11x11 box), 3D tile-shape index (0..3 one sweep per pass; 4: per grid,
+ * the better of shapes 2 and 3 by width; 5, 6: two sweeps per pass) and z-segment
+ * length, and 9 scheduling
  * entries: queue affinity of the 2D and of the 3D launches (1: waves start at
  * their own XCD's queue, 0: all waves work through the tiles in table order),
  * whether 3D sweeps alternate direction (odd steps walk the table back to front),

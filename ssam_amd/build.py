@@ -45,6 +45,7 @@ SOURCES = [
     "batch.hip",
     "batch2d.hip",
     "batch3d.hip",
+    "batch3d_t2.hip",
     "microbench.hip",
 ]
 

@@ -1,27 +1,26 @@
-// batch.hip -- many grids of uneven sizes and mixed stencil shapes in one
-// persistent launch fed by tile work queues (2D), multi-step 3D sweeps in one
-// persistent launch per step, and the whole sequence captured in a hipGraph.
+// batch.hip -- many grids of uneven sizes and mixed stencil shapes: the plan,
+// its tile tables, the launch sequence captured in a hipGraph, and the tuner
+// (the kernels are in batch2d.hip and batch3d.hip).
 //
 // No counterpart in the reference (one grid, one launch per harness call).
 //
 //  * Plan: every 2D item is cut into tiles (column strip x segment of rows) of the
 //    same geometry the single-grid launch uses; 3D items into tiles of (x-strip x
-//    y-block x z-segment).  Each item's tiles stay in memory order (neighbouring
-//    tiles share halo rows) and are cut into runs; the runs are packed into 8
-//    queues of equal estimated cost, one per XCD, with the most expensive tiles of
-//    each queue first.
-//  * Kernel: a fixed grid of as many waves as the device holds at once; a wave
-//    reads its XCD id and takes the next tile of that XCD's queue (one vector
-//    atomic from lane 0), so neighbouring tiles run under one L2 and share their
-//    halo rows there; when its queue is empty it takes tiles from the other queues
-//    in turn (the result never depends on which XCD runs what), and leaves when all
-//    are empty.  With affinity off every wave starts at queue 0 and the device
-//    works through the table in order (the tuner picks per launch kind).
-//  * Graph: counter reset (a fill kernel, no memset), the 2D launches, the per-step
-//    3D launches and any per-item fallback launches are captured once on a private
+//    y-block x z-segment), one or two sweeps per pass.
+//  * Launch forms: an ordinary grid with one tile per wave over a table sorted by
+//    decreasing tile cost (the hardware workgroup dispatcher is the queue; the
+//    default), or a persistent grid of as many waves as the device holds, which
+//    drains 8 work queues (one per XCD, each item's tiles in memory order, cut
+//    into runs packed to equal cost): a wave reads its XCD id, takes the next tile
+//    of that queue (one atomic from lane 0), steals from the other queues when its
+//    own is empty, and leaves when all are empty; the result never depends on which
+//    XCD runs what.  2D items run one launch per stencil kind or one for all kinds.
+//  * Graph: counter reset (a fill kernel, no memset), the 2D launches, the 3D
+//    passes and any per-item fallback launches are captured once on a private
 //    stream and replayed on the caller's stream.
-//  * Tuner: ssam_batch_plan_tune times candidate tile heights per 2D shape, 3D
-//    tile shapes and queue affinity on the plan's own buffers, keeps the fastest.
+//  * Tuner: ssam_batch_plan_tune times launch forms, tile heights per 2D shape,
+//    3D tile shapes (temporal degree included), occupancy and queue affinity on
+//    the plan's own buffers and keeps the fastest.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,7 +45,7 @@ using namespace ssam;
 // ------------------------------------------------------------------ plan
 struct ssam_batch_plan_s {
     int device = 0;
-    struct Group {  // one persistent launch (2D) or one launch per step (3D)
+    struct Group {  // 2D: one launch per kind or for all kinds; 3D: one launch per pass
         int ndim = 2;
         int storage = 0;
         std::vector<int> ids;  // indices into items
@@ -77,7 +76,7 @@ struct ssam_batch_plan_s {
     int split2d = 1;           // 2D: one launch per kind (1) or one for all kinds (0)
     int mfma2d = 1;            // BF16/FP16 5x5 and 11x11 boxes on the matrix cores
     int wps2d = 0, wps3d = 1;  // persistent grid: waves per SIMD (0: as many as fit)
-    int grid2d = 0, grid3d = 0;  // one tile per wave of an ordinary grid (1) or persistent (0)
+    int grid2d = 1, grid3d = 1;  // one tile per wave of an ordinary grid (1) or persistent (0)
     int ncounters = 0;
     void* d_counters = nullptr;
     int64_t tiles2d = 0, tiles3d = 0, point_updates = 0;
@@ -89,6 +88,9 @@ struct ssam_batch_plan_s {
 namespace {
 
 using Plan = ssam_batch_plan_s;
+
+// 3D launch form in effect: the two-sweep shape always runs as an ordinary grid.
+bool grid3d_on(const Plan* P) { return P->grid3d || kVar3D[P->var3d].t == 2; }
 
 int elem_size_b(int dtype) {
     switch (dtype) {
@@ -384,13 +386,13 @@ int build_group3d(Plan* P, Plan::Group& g) {
         }
     }
     std::vector<int4> tiles;
-    if (P->grid3d) pack_single(per, &tiles, &g.qs);
+    if (grid3d_on(P)) pack_single(per, &tiles, &g.qs);
     else pack_queues(per, &tiles, &g.qs);
     g.qs.affinity = P->aff3d;
     g.qs.alternate = P->alt3d;
     free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
-    g.blocks = P->grid3d ? (g.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock
+    g.blocks = grid3d_on(P) ? (g.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock
                          : resident_blocks(batch3d_kernel_fn(g.storage, variant, g.kind, false), P->wps3d);
     int rc = upload(&g.d_items, h);
     if (rc == SSAM_OK) rc = upload(&g.d_tiles, tiles);
@@ -516,7 +518,7 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             int step = s;
             const void* taps = g.d_taps;
             void* args[] = {&items, &tiles, &qs, &cnt, &taps, &step, &single};
-            const void* fn = batch3d_kernel_fn(g.storage, P->var3d, g.kind, P->grid3d != 0);
+            const void* fn = batch3d_kernel_fn(g.storage, P->var3d, g.kind, grid3d_on(P));
             if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
                     hipSuccess ||
                 hipGetLastError() != hipSuccess)
@@ -629,8 +631,8 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
     P->mfma2d = env_int("SSAM_BATCH2D_MFMA", 1) != 0;
     P->wps2d = std::max(0, env_int("SSAM_BATCH2D_WPS", 0));
     P->wps3d = std::max(0, env_int("SSAM_BATCH3D_WPS", 1));
-    P->grid2d = env_int("SSAM_BATCH2D_GRID", 0) != 0;
-    P->grid3d = env_int("SSAM_BATCH3D_GRID", 0) != 0;
+    P->grid2d = env_int("SSAM_BATCH2D_GRID", 1) != 0;
+    P->grid3d = env_int("SSAM_BATCH3D_GRID", 1) != 0;
     P->aff3d = env_int("SSAM_BATCH3D_AFFINITY", 0) != 0;
     P->alt3d = env_int("SSAM_BATCH3D_ALTERNATE", 1) != 0;
     for (int s = 0; s < 4; ++s) {
@@ -755,8 +757,8 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
         // tile shape x z-segment length x launch form (persistent / grid), then
         // the persistent grid's occupancy, queue affinity and step direction
         search({&P->var3d, &P->zseg3d, &P->grid3d},
-               {{0, 1, 2, 3, 4, 5, 6, 7, kVarPair3D, 9, 10, 11, 12}, {16, 32, 64}, {0, 1}}, kRun3D);
-        if (!P->grid3d) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
+               {{0, 1, 2, 3, kVarPair3D, kVarT2}, {16, 32, 64}, {0, 1}}, kRun3D);
+        if (!grid3d_on(P)) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
         search({&P->aff3d, &P->alt3d}, {{0, 1}, {0, 1}}, kRun3D);
     }
     const int rc2 = build_all(P);
@@ -774,7 +776,7 @@ int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* 
         sched[4] = plan->mfma2d;
         sched[5] = plan->wps2d;
         sched[6] = plan->wps3d;
-        sched[7] = plan->grid3d;
+        sched[7] = grid3d_on(plan) ? 1 : 0;
         sched[8] = plan->grid2d;
     }
     if (seg_rows_2d)

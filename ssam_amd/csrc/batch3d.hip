@@ -1,73 +1,11 @@
-// batch3d.hip -- the 3D kernels of the batched launch: one sweep of radius-1
-// stars / face+edge stencils over a table of (item, strip, y-block, z-segment)
-// tiles, from a persistent grid draining the XCD queues or an ordinary grid with
-// one tile per wave.
-#include "batch_common.hpp"
+// batch3d.hip -- the 3D kernels of the batched launch with one sweep per pass:
+// radius-1 stars / face+edge stencils over a table of (item, strip, y-block,
+// z-segment) tiles, from a persistent grid draining the XCD queues or an
+// ordinary grid with one tile per wave.  The two-sweep shape is in batch3d_t2.hip.
+#include "batch3d_kernel.hpp"
 
 namespace ssam {
 
-// tile.x item, .y strip, .z y-block, .w first plane (low 20 bits) | planes << 20
-// GRID: one tile per wave of an ordinary grid over the table (the hardware's
-// workgroup dispatch is the queue: no atomics, no drain loop); otherwise a
-// persistent grid draining the XCD queues.
-template <typename S, int VAR> struct Waves3D { static constexpr int value = Var3DT<S, VAR>::WPE; };
-template <typename S> struct Waves3D<S, kVarPair3D> {
-    static constexpr int a = Var3DT<S, kPairA>::WPE, b = Var3DT<S, kPairB>::WPE;
-    static constexpr int value = a < b ? a : b;
-};
-
-template <typename S, int SHAPE, int VAR>
-__device__ __forceinline__ void run_tile3d(const S* in, S* out, const BatchItem3D& b, const S* w, int strip,
-                                           int yb, int zs, int ze, bool single) {
-    using T = Var3DT<S, VAR>;
-    if constexpr (T::T == 2) {
-        Stencil3DHL2<S, SHAPE, T::V, T::NY, T::PF>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze,
-                                                              single);
-    } else {
-        (void)single;
-        Stencil3DHL<S, SHAPE, T::V, T::NY, T::PF>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze);
-    }
-}
-
-// VAR == kVarPair3D: each item runs the tile shape the plan chose for it.
-template <typename S, int VAR, int SHAPE, bool GRID>
-__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(Waves3D<S, VAR>::value)))
-void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
-                    int* __restrict__ counters, const S* __restrict__ taps, int step, int single) {
-    auto body = [&](int t) {
-        const int4 e = tiles[t];
-        const int it = __builtin_amdgcn_readfirstlane(e.x);
-        const int strip = __builtin_amdgcn_readfirstlane(e.y);
-        const int yb = __builtin_amdgcn_readfirstlane(e.z);
-        const int zw = __builtin_amdgcn_readfirstlane(e.w);
-        const int zs = zw & 0xFFFFF, ze = zs + (zw >> 20);
-        const BatchItem3D& b = items[it];
-        // passes ping-pong so the last pass writes dst (same rule as ssam_stencil3d
-        // with passes for sweeps); b.steps holds the item's pass count
-        const void* in = step == 0 ? b.src : (((b.steps - step) % 2 == 0) ? b.dst : b.work);
-        void* out = ((b.steps - 1 - step) % 2 == 0) ? b.dst : b.work;
-        const S* src = static_cast<const S*>(in);
-        S* dst = static_cast<S*>(out);
-        const S* w = taps + b.taps_off;
-        if constexpr (VAR == kVarPair3D) {
-            if (b.var == kPairB) run_tile3d<S, SHAPE, kPairB>(src, dst, b, w, strip, yb, zs, ze, false);
-            else run_tile3d<S, SHAPE, kPairA>(src, dst, b, w, strip, yb, zs, ze, false);
-        } else {
-            run_tile3d<S, SHAPE, VAR>(src, dst, b, w, strip, yb, zs, ze, single != 0);
-        }
-    };
-    if constexpr (GRID) {
-        (void)counters;
-        const int n = qs.beg[kQueues];
-        const int t = blockIdx.x * kBatchWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-        if (t < n) body(qs.alternate && (step & 1) ? n - 1 - t : t);
-    } else {
-        drain_queues<false>(qs, counters, qs.alternate && (step & 1), body);
-    }
-}
-
-template <typename S>
-using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, const S*, int, int);
 
 template <typename S, int SHAPE, bool GRID> Batch3DFn<S> batch3d_fn_shape(int variant) {
     switch (variant) {
@@ -75,14 +13,6 @@ template <typename S, int SHAPE, bool GRID> Batch3DFn<S> batch3d_fn_shape(int va
         case 1: return batch3d_kernel<S, 1, SHAPE, GRID>;
         case 2: return batch3d_kernel<S, 2, SHAPE, GRID>;
         case 3: return batch3d_kernel<S, 3, SHAPE, GRID>;
-        case 4: return batch3d_kernel<S, 4, SHAPE, GRID>;
-        case 5: return batch3d_kernel<S, 5, SHAPE, GRID>;
-        case 6: return batch3d_kernel<S, 6, SHAPE, GRID>;
-        case 7: return batch3d_kernel<S, 7, SHAPE, GRID>;
-        case 9: return batch3d_kernel<S, 9, SHAPE, GRID>;
-        case 10: return batch3d_kernel<S, 10, SHAPE, GRID>;
-        case 11: return batch3d_kernel<S, 11, SHAPE, GRID>;
-        case 12: return batch3d_kernel<S, 12, SHAPE, GRID>;
         default: return batch3d_kernel<S, kVarPair3D, SHAPE, GRID>;
     }
 }
@@ -97,6 +27,7 @@ template <typename S> const void* batch3d_fn_t(int variant, int shape3d, bool gr
 }
 
 const void* batch3d_kernel_fn(int storage, int variant, int shape3d, bool grid) {
+    if (kVar3D[variant].t == 2) return batch3d_t2_kernel_fn(storage, shape3d);
     return storage == SSAM_F64 ? batch3d_fn_t<double>(variant, shape3d, grid)
                                : batch3d_fn_t<float>(variant, shape3d, grid);
 }

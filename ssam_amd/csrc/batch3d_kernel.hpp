@@ -1,0 +1,72 @@
+// batch3d_kernel.hpp -- the batched 3D kernel template (instantiated by
+// batch3d.hip for the one-sweep tile shapes and by batch3d_t2.hip for the
+// two-sweep shape, so the two compile in parallel).
+#pragma once
+#include "batch_common.hpp"
+
+namespace ssam {
+
+// tile.x item, .y strip, .z y-block, .w first plane (low 20 bits) | planes << 20
+// GRID: one tile per wave of an ordinary grid over the table (the hardware's
+// workgroup dispatch is the queue: no atomics, no drain loop); otherwise a
+// persistent grid draining the XCD queues.
+template <typename S, int VAR> struct Waves3D { static constexpr int value = Var3DT<S, VAR>::WPE; };
+template <typename S> struct Waves3D<S, kVarPair3D> {
+    static constexpr int a = Var3DT<S, kPairA>::WPE, b = Var3DT<S, kPairB>::WPE;
+    static constexpr int value = a < b ? a : b;
+};
+
+template <typename S, int SHAPE, int VAR>
+__device__ __forceinline__ void run_tile3d(const S* in, S* out, const BatchItem3D& b, const S* w, int strip,
+                                           int yb, int zs, int ze, bool single) {
+    using T = Var3DT<S, VAR>;
+    if constexpr (T::T == 2) {
+        Stencil3DHL2<S, SHAPE, T::V, T::NY, T::PF>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze,
+                                                              single);
+    } else {
+        (void)single;
+        Stencil3DHL<S, SHAPE, T::V, T::NY, T::PF>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze);
+    }
+}
+
+// VAR == kVarPair3D: each item runs the tile shape the plan chose for it.
+template <typename S, int VAR, int SHAPE, bool GRID>
+__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(Waves3D<S, VAR>::value)))
+void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
+                    int* __restrict__ counters, const S* __restrict__ taps, int step, int single) {
+    auto body = [&](int t) {
+        const int4 e = tiles[t];
+        const int it = __builtin_amdgcn_readfirstlane(e.x);
+        const int strip = __builtin_amdgcn_readfirstlane(e.y);
+        const int yb = __builtin_amdgcn_readfirstlane(e.z);
+        const int zw = __builtin_amdgcn_readfirstlane(e.w);
+        const int zs = zw & 0xFFFFF, ze = zs + (zw >> 20);
+        const BatchItem3D& b = items[it];
+        // passes ping-pong so the last pass writes dst (same rule as ssam_stencil3d
+        // with passes for sweeps); b.steps holds the item's pass count
+        const void* in = step == 0 ? b.src : (((b.steps - step) % 2 == 0) ? b.dst : b.work);
+        void* out = ((b.steps - 1 - step) % 2 == 0) ? b.dst : b.work;
+        const S* src = static_cast<const S*>(in);
+        S* dst = static_cast<S*>(out);
+        const S* w = taps + b.taps_off;
+        if constexpr (VAR == kVarPair3D) {
+            if (b.var == kPairB) run_tile3d<S, SHAPE, kPairB>(src, dst, b, w, strip, yb, zs, ze, false);
+            else run_tile3d<S, SHAPE, kPairA>(src, dst, b, w, strip, yb, zs, ze, false);
+        } else {
+            run_tile3d<S, SHAPE, VAR>(src, dst, b, w, strip, yb, zs, ze, single != 0);
+        }
+    };
+    if constexpr (GRID) {
+        (void)counters;
+        const int n = qs.beg[kQueues];
+        const int t = blockIdx.x * kBatchWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+        if (t < n) body(qs.alternate && (step & 1) ? n - 1 - t : t);
+    } else {
+        drain_queues<false>(qs, counters, qs.alternate && (step & 1), body);
+    }
+}
+
+template <typename S>
+using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, const S*, int, int);
+
+}  // namespace ssam

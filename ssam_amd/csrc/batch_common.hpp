@@ -151,16 +151,18 @@ template <typename S> constexpr bool kHalf2D = std::is_same<S, bf16_t>::value ||
 struct Var3D { int v, ny, pf, wpe, t; };
 // Variant index kVarPair3D: per item, the better of shapes kPairA (narrow
 // grids) and kPairB (wider ones), in one kernel (pair_choice below); its table
-// entry repeats kPairA's shape.
-constexpr int kVarPair3D = 8;
-constexpr int kPairA = 5, kPairB = 7;
-constexpr Var3D kVar3D[] = {{1, 8, 2, 4, 1},  {2, 16, 1, 2, 1}, {2, 4, 2, 4, 1}, {2, 8, 1, 3, 1},
-                            {2, 8, 2, 2, 1},  {2, 12, 2, 2, 1}, {1, 16, 2, 2, 1}, {4, 8, 1, 2, 1},
-                            {2, 12, 2, 2, 1}, {2, 8, 1, 2, 2},  {2, 6, 1, 3, 2},  {4, 4, 1, 2, 2},
-                            {1, 16, 1, 2, 2}};
-constexpr int kNumVar3D = 13;
-constexpr int kDefaultVar3D = 4;
-constexpr int kDefaultZseg3D = 32;
+// entry repeats kPairA's shape.  The shapes kept are those that measured within
+// reach of the best on MI355X (a wider menu only lengthened the build).
+constexpr int kVarPair3D = 4;
+constexpr int kPairA = 2, kPairB = 3;
+// kVarT2: two sweeps per pass, 2 columns x 6 rows per lane (the fastest on the
+// uneven batch; always an ordinary grid)
+constexpr int kVarT2 = 5;
+constexpr Var3D kVar3D[] = {{1, 8, 2, 4, 1}, {2, 8, 2, 2, 1}, {2, 12, 2, 2, 1}, {4, 8, 1, 2, 1},
+                            {2, 12, 2, 2, 1}, {2, 6, 1, 3, 2}};
+constexpr int kNumVar3D = 6;
+constexpr int kDefaultVar3D = kVarT2;
+constexpr int kDefaultZseg3D = 16;
 constexpr int kDefaultSeg2D = 64;
 
 template <typename S, int VAR> struct Var3DT {
@@ -200,5 +202,6 @@ inline int pair_choice(long long nx, int elem_bytes) {
 // grid: one tile per wave of an ordinary grid instead of a persistent grid.
 const void* batch2d_kernel_fn(int storage, int kind, bool grid);
 const void* batch3d_kernel_fn(int storage, int variant, int shape3d, bool grid);
+const void* batch3d_t2_kernel_fn(int storage, int shape3d);
 
 }  // namespace ssam
