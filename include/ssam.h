@@ -72,7 +72,9 @@ typedef enum {
 typedef struct {
     int32_t storage;      /* ssam_dtype of src, dst and work */
     int32_t shape;        /* STAR, BOX or DENSE */
-    int32_t fw, fh;       /* filter width / height; STAR and BOX: fw == fh == 2r+1 */
+    int32_t fw, fh;       /* filter width / height; STAR and BOX: fw == fh == 2r+1;
+                             STAR r <= 6, BOX r <= 14, DENSE up to 29 x 29
+                             (COMPENSATED: up to 19 x 19) */
     int32_t steps;        /* time steps applied (>= 1); > 1 only for ZERO boundary */
     int32_t boundary;     /* ssam_boundary: ZERO, CLAMP or HOLD */
     uint32_t flags;       /* SSAM_FLAG_* (other bits: SSAM_E_INVALID) */
@@ -126,7 +128,7 @@ int ssam_j2dstar_f32(const float* src, float* dst, int width, int height, int ra
 int ssam_j2dstar_f64(const double* src, double* dst, int width, int height, int radius,
                      const double* coefs, void* stream);
 
-/* Box of radius r (j2d64pt / j2d81pt / j2d121pt, stencil2D_64pt.cu:74,
+/* Box of radius r = 1..14 (j2d64pt / j2d81pt / j2d121pt, stencil2D_64pt.cu:74,
  * stencil2D_81pt.cu:92, stencil2D_121pt.cu:132, which bake their taps in as
  * macros; here they are a host array w[(2r+1)^2], row-major). */
 int ssam_j2dbox_f32(const float* src, float* dst, int width, int height, int radius,
@@ -152,10 +154,13 @@ int ssam_j2d9pt_temporal_f64(const double* src, double* dst, int width, int heig
                              void* stream);
 
 /* CONV2D::kernel_convolution2D(src, dst, width, widthStride, height, weight)
- * (ssai-2dconv/conv2D.cu:9-12): FW x FH correlation, zero padding, pitched rows.
- * `weight` is DEVICE memory (fh*fw, row-major) as in the reference; it is read
- * back once per call, so this entry SYNCHRONISES `stream`.  Use ssam_stencil2d
- * with SSAM_SHAPE_DENSE for an asynchronous, graph-capturable call. */
+ * (ssai-2dconv/conv2D.cu:9-12): FW x FH correlation, zero padding, pitched rows,
+ * filter sizes up to 28 x 28 (the reference dispatches 2..28, conv2D.cu:186-212;
+ * 29 x 29 is accepted too).  `weight` is DEVICE memory (fh*fw, row-major) as in the
+ * reference.  Square filters (fw == fh >= 2) are read by the kernel in place: the
+ * call is asynchronous and graph-capturable, and `weight` must stay valid until
+ * the stream has run it.  Other shapes (and 1 x 1) are read back once per call
+ * and embedded on the host, so those calls SYNCHRONISE `stream`. */
 int ssam_conv2d_f32(const float* src, float* dst, int width, int width_stride, int height,
                     const float* weight, int fw, int fh, void* stream);
 int ssam_conv2d_f64(const double* src, double* dst, int width, int width_stride, int height,
@@ -263,8 +268,12 @@ void ssam_batch_plan_destroy(ssam_batch_plan plan);
  *     2 ds_bpermute + add, 3 LDS store + load round trip + add;
  *   which 4..8: wave-instructions per cycle per CU over the whole device, 8 waves
  *     per CU with 8 independent chains each -- 4 FMA, 5 DPP add, 6 ds_bpermute,
- *     7 LDS round trip, 8 v_mfma_f32_16x16x32_bf16.
- * Cycles are shader clocks (s_memtime).  Synchronises `stream`. */
+ *     7 LDS round trip, 8 v_mfma_f32_16x16x32_bf16;
+ *   which 9..13: device rate in TFLOP/s, 32 waves per CU with 8 independent
+ *     chains, timed with HIP events (wall clock) -- 9 FP32 FMA (no packed FP32),
+ *     10 FP64 FMA, 11 v_mfma_f32_16x16x4_f32, 12 v_mfma_f32_4x4x1_16b_f32,
+ *     13 v_mfma_f64_16x16x4_f64.
+ * Cycles (0..8) are s_memtime counts.  Synchronises `stream`. */
 int ssam_microbench(int which, double* result, void* stream);
 
 /* ---------------------------------------------------------------- autotuner

@@ -174,13 +174,15 @@ def j2d9pt_temporal(src, dst, c9: Sequence[float], steps: int, work=None, stream
 
 def conv2d(src, dst, weight: torch.Tensor, stream=None) -> None:
     """CONV2D::kernel_convolution2D (conv2D.cu:9-12): pitched rows, DEVICE weights
-    (fh, fw); synchronises the stream to read the weights."""
+    (fh, fw).  Square filters are read by the kernel in place (asynchronous, so the
+    weights must stay alive until the stream has run the call); other shapes are
+    read back, which synchronises the stream."""
     w, h, pitch = _check_2d(src, dst)
-    if weight.dim() != 2 or weight.dtype != src.dtype:
-        raise ValueError("weight must be a 2-D device tensor of the grid dtype")
+    if weight.dim() != 2 or weight.dtype != src.dtype or not weight.is_contiguous():
+        raise ValueError("weight must be a contiguous 2-D device tensor of the grid dtype")
     fh, fw = weight.shape
     fn_ = getattr(L.lib(), f"ssam_conv2d_{_sfx(src)}")
-    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, pitch, h, _dev(weight.contiguous(), "weight"),
+    check(fn_(_dev(src, "src"), _dev(dst, "dst"), w, pitch, h, _dev(weight, "weight"),
               fw, fh, _stream(stream)), "ssam_conv2d")
 
 
@@ -325,7 +327,9 @@ class BatchPlan:
 
 MICROBENCH = ("fma_latency_cycles", "dpp_add_latency_cycles", "bpermute_latency_cycles",
               "lds_roundtrip_latency_cycles", "fma_per_cycle_per_cu", "dpp_add_per_cycle_per_cu",
-              "bpermute_per_cycle_per_cu", "lds_roundtrip_per_cycle_per_cu", "mfma_bf16_16x16x32_per_cycle_per_cu")
+              "bpermute_per_cycle_per_cu", "lds_roundtrip_per_cycle_per_cu", "mfma_bf16_16x16x32_per_cycle_per_cu",
+              "fma_f32_tflops", "fma_f64_tflops", "mfma_f32_16x16x4_tflops", "mfma_f32_4x4x1_tflops",
+              "mfma_f64_16x16x4_tflops")
 
 
 def microbench(stream: Optional[torch.cuda.Stream] = None) -> dict:

@@ -41,6 +41,15 @@ SOURCES = [
     "stencil2d_f64.hip",
     "stencil2d_bf16.hip",
     "stencil2d_f16.hip",
+    "stencil2d_box_f32.hip",
+    "stencil2d_box_f64.hip",
+    "stencil2d_box_bf16.hip",
+    "stencil2d_box_f16.hip",
+    "stencil2d_even_f32.hip",
+    "stencil2d_even_f64.hip",
+    "stencil2d_kahan_f32.hip",
+    "stencil2d_kahan_bf16.hip",
+    "stencil2d_kahan_f16.hip",
     "stencil3d.hip",
     "batch.hip",
     "batch2d.hip",

@@ -20,7 +20,7 @@ namespace ssam {
 namespace {
 
 constexpr int kMaxStarRadius = 6;
-constexpr int kMaxBoxRadius = 9;
+constexpr int kMaxBoxRadius = 14;  // 29 x 29 box: dense filters up to 28 x 28 embed in it
 constexpr int kMaxFusedSteps = 8;
 
 int elem_size(int dtype) {
@@ -105,71 +105,35 @@ thread_local int t_force_wps = 0;
 thread_local int t_force_tchunk = 0;
 thread_local int t_force_form = 0;
 
-int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int64_t width,
-                   int64_t height, int64_t pitch, void* work, hipStream_t st) {
-    if (!d || !src || !dst || !d->coeffs) return SSAM_E_INVALID;
-    const int es = elem_size(d->storage);
-    if (!es) return SSAM_E_INVALID;
+// Where a launch sequence takes its taps from: a host (2R+1)^2 box in FP64, or the
+// caller's device array in the accumulation type, read in place (Req2D::dev_taps).
+struct TapSrc {
+    const double* host;
+    const void* dev;
+};
+
+// Sizes, aliasing, the split of `steps` into fused launches (8, 4, 2, 1, or the
+// tuned chunk) chained through work/dst, and the launches.
+int stencil2d_run(int storage, int kshape, int R, int steps, int boundary, const TapSrc& ts,
+                  const void* src, void* dst, int64_t width, int64_t height, int64_t pitch,
+                  void* work, hipStream_t st) {
+    const int es = elem_size(storage);
+    if (!es || !src || !dst) return SSAM_E_INVALID;
     if (width < 1 || height < 1 || pitch < width) return SSAM_E_INVALID;
     if (width > (1LL << 30) || height > (1LL << 30) || pitch > (1LL << 31))
         return SSAM_E_UNSUPPORTED;
-    if (d->steps < 1) return SSAM_E_INVALID;
-    if (d->boundary != SSAM_BOUNDARY_ZERO && d->boundary != SSAM_BOUNDARY_CLAMP &&
-        d->boundary != SSAM_BOUNDARY_HOLD)
-        return SSAM_E_INVALID;
-    if (d->boundary != SSAM_BOUNDARY_ZERO && d->steps != 1) return SSAM_E_INVALID;
-    if (d->fw < 1 || d->fh < 1) return SSAM_E_INVALID;
-    if (d->flags & ~SSAM_FLAG_COMPENSATED) return SSAM_E_INVALID;
-    const bool comp = (d->flags & SSAM_FLAG_COMPENSATED) != 0;
-    if (comp && (d->shape == SSAM_SHAPE_STAR || d->steps != 1)) return SSAM_E_UNSUPPORTED;
-
-    // Embed the filter in a (2R+1)^2 box of taps centred like the reference's
-    // correlation (offset n - fw/2, m - fh/2; common.h:196-197).
-    int shape = d->shape;
-    int R = 0;
-    if (shape == SSAM_SHAPE_STAR || shape == SSAM_SHAPE_BOX) {
-        if (d->fw != d->fh || d->fw % 2 == 0 || d->fw < 3) return SSAM_E_INVALID;
-        R = d->fw / 2;
-        if (R > (shape == SSAM_SHAPE_STAR ? kMaxStarRadius : kMaxBoxRadius))
-            return SSAM_E_UNSUPPORTED;
-    } else if (shape == SSAM_SHAPE_DENSE) {
-        R = d->fw / 2 > d->fh / 2 ? d->fw / 2 : d->fh / 2;
-        if (R < 1) R = 1;
-        if (R > kMaxBoxRadius) return SSAM_E_UNSUPPORTED;
-    } else {
-        return SSAM_E_INVALID;
-    }
-    const int K = 2 * R + 1;
-    std::vector<double> taps(static_cast<size_t>(K) * K, 0.0);
-    if (shape == SSAM_SHAPE_DENSE) {
-        for (int m = 0; m < d->fh; ++m)
-            for (int n = 0; n < d->fw; ++n) {
-                const int dy = m - d->fh / 2, dx = n - d->fw / 2;
-                taps[static_cast<size_t>(dy + R) * K + (dx + R)] = d->coeffs[m * d->fw + n];
-            }
-    } else {
-        for (int i = 0; i < K * K; ++i) taps[i] = d->coeffs[i];
-        if (shape == SSAM_SHAPE_STAR) {
-            for (int m = 0; m < K; ++m)
-                for (int n = 0; n < K; ++n)
-                    if (m != R && n != R) taps[static_cast<size_t>(m) * K + n] = 0.0;
-        }
-    }
-    // DENSE runs as a box; compensated boxes are their own kernel instance
-    const int kshape = shape == SSAM_SHAPE_STAR ? kStar : (comp ? kBoxKahan : kBox);
-
     const int64_t bytes = ((height - 1) * pitch + width) * es;
     if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
 
-    // split the steps into fused launches (8, 4, 2, 1) chained through work/dst
-    const TuneKey key{d->storage, kshape, R, d->steps, width, height};
+    const int shape = kshape == kStar ? SSAM_SHAPE_STAR : SSAM_SHAPE_BOX;
+    const TuneKey key{storage, kshape, R, steps, width, height};
     TuneVal tv{kDefaultWavesPerSimd, 0, 0.f, kFormAuto};
     const bool tuned = tune_lookup(key, &tv);
-    int fmax = max_fused(d->storage, shape, R);
+    int fmax = max_fused(storage, shape, R);
     if (t_force_tchunk > 0) fmax = t_force_tchunk < fmax ? t_force_tchunk : fmax;
     else if (tuned && tv.tchunk > 0 && tv.tchunk < fmax) fmax = tv.tchunk;
     std::vector<int> chunks;
-    for (int left = d->steps; left > 0;) {
+    for (int left = steps; left > 0;) {
         int t = fmax;
         while (t > left) t >>= 1;
         chunks.push_back(t);
@@ -187,8 +151,9 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
     rq.pitch = pitch;
     rq.shape = kshape;
     rq.radius = R;
-    rq.boundary = d->boundary;
-    rq.taps = taps.data();
+    rq.boundary = boundary;
+    rq.taps = ts.host;
+    rq.dev_taps = ts.dev;
     rq.seg_rows = env_int("SSAM_SEG_ROWS", 0);
     rq.waves_per_simd = t_force_wps > 0 ? t_force_wps
                         : env_int("SSAM_WAVES_PER_SIMD", tuned ? tv.waves_per_simd
@@ -202,10 +167,122 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
         rq.src = in;
         rq.dst = out;
         rq.steps = chunks[i];
-        const int rc = launch2d(d->storage, rq, st);
+        const int rc = launch2d(storage, rq, st);
         if (rc != SSAM_OK) return rc;
         in = out;
     }
+    return SSAM_OK;
+}
+
+// The kernel shape, radius and host taps of a descriptor (validated): STAR and
+// BOX keep their (2r+1)^2 taps; DENSE filters are embedded in a (2R+1)^2 box
+// centred like the reference's correlation (offset n - fw/2, m - fh/2;
+// common.h:196-197), except even square ones with F32/F64 storage, which keep
+// their (2r)^2 taps as kBoxEven.  Compensated boxes are their own kernel shape.
+int kernel_form(const ssam_stencil2d_desc* d, int* kshape, int* radius, std::vector<double>* taps) {
+    if (!d || !d->coeffs) return SSAM_E_INVALID;
+    if (!elem_size(d->storage)) return SSAM_E_INVALID;
+    if (d->steps < 1) return SSAM_E_INVALID;
+    if (d->boundary != SSAM_BOUNDARY_ZERO && d->boundary != SSAM_BOUNDARY_CLAMP &&
+        d->boundary != SSAM_BOUNDARY_HOLD)
+        return SSAM_E_INVALID;
+    if (d->boundary != SSAM_BOUNDARY_ZERO && d->steps != 1) return SSAM_E_INVALID;
+    if (d->fw < 1 || d->fh < 1) return SSAM_E_INVALID;
+    if (d->flags & ~SSAM_FLAG_COMPENSATED) return SSAM_E_INVALID;
+    const bool comp = (d->flags & SSAM_FLAG_COMPENSATED) != 0;
+    if (comp && (d->shape == SSAM_SHAPE_STAR || d->steps != 1)) return SSAM_E_UNSUPPORTED;
+
+    const int shape = d->shape;
+    int R = 0;
+    if (shape == SSAM_SHAPE_STAR || shape == SSAM_SHAPE_BOX) {
+        if (d->fw != d->fh || d->fw % 2 == 0 || d->fw < 3) return SSAM_E_INVALID;
+        R = d->fw / 2;
+        if (R > (shape == SSAM_SHAPE_STAR ? kMaxStarRadius : kMaxBoxRadius))
+            return SSAM_E_UNSUPPORTED;
+    } else if (shape == SSAM_SHAPE_DENSE) {
+        R = d->fw / 2 > d->fh / 2 ? d->fw / 2 : d->fh / 2;
+        if (R < 1) R = 1;
+        if (R > kMaxBoxRadius) return SSAM_E_UNSUPPORTED;
+    } else {
+        return SSAM_E_INVALID;
+    }
+    if (comp && R > 9) return SSAM_E_UNSUPPORTED;  // compensated kernels go to 19 x 19
+    const int K = 2 * R + 1;
+    const bool even = shape == SSAM_SHAPE_DENSE && !comp && d->fw == d->fh && d->fw % 2 == 0 &&
+                      (d->storage == SSAM_F32 || d->storage == SSAM_F64);
+    if (even) {
+        taps->assign(d->coeffs, d->coeffs + static_cast<size_t>(d->fw) * d->fw);
+        *kshape = kBoxEven;
+        *radius = R;
+        return SSAM_OK;
+    }
+    taps->assign(static_cast<size_t>(K) * K, 0.0);
+    if (shape == SSAM_SHAPE_DENSE) {
+        for (int m = 0; m < d->fh; ++m)
+            for (int n = 0; n < d->fw; ++n) {
+                const int dy = m - d->fh / 2, dx = n - d->fw / 2;
+                (*taps)[static_cast<size_t>(dy + R) * K + (dx + R)] = d->coeffs[m * d->fw + n];
+            }
+    } else {
+        for (int i = 0; i < K * K; ++i) (*taps)[i] = d->coeffs[i];
+        if (shape == SSAM_SHAPE_STAR) {
+            for (int m = 0; m < K; ++m)
+                for (int n = 0; n < K; ++n)
+                    if (m != R && n != R) (*taps)[static_cast<size_t>(m) * K + n] = 0.0;
+        }
+    }
+    *kshape = shape == SSAM_SHAPE_STAR ? kStar : (comp ? kBoxKahan : kBox);
+    *radius = R;
+    return SSAM_OK;
+}
+
+int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int64_t width,
+                   int64_t height, int64_t pitch, void* work, hipStream_t st) {
+    int kshape = 0, R = 0;
+    std::vector<double> taps;
+    const int rc = kernel_form(d, &kshape, &R, &taps);
+    if (rc != SSAM_OK) return rc;
+    const TapSrc ts{taps.data(), nullptr};
+    return stencil2d_run(d->storage, kshape, R, d->steps, d->boundary, ts, src, dst, width, height,
+                         pitch, work, st);
+}
+
+// ------------------------------------------------------------------ device tap copies
+// Keyed by (bytes, contents); entries live until the process ends (at most
+// kMaxTapCopies per device: past that the cache is emptied after a device-wide
+// synchronisation, so no launch can still be reading a freed copy).
+struct TapCopy {
+    std::vector<unsigned char> key;
+    void* ptr;
+};
+std::mutex g_taps_mu;
+std::vector<TapCopy> g_tap_copies[64];
+constexpr size_t kMaxTapCopies = 64;
+
+int device_tap_copy(const void* host, size_t bytes, const void** dev) {
+    int devid = 0;
+    if (hipGetDevice(&devid) != hipSuccess || devid < 0 || devid >= 64) return SSAM_E_HIP;
+    const unsigned char* h = static_cast<const unsigned char*>(host);
+    std::lock_guard<std::mutex> lk(g_taps_mu);
+    auto& v = g_tap_copies[devid];
+    for (const auto& e : v)
+        if (e.key.size() == bytes && std::memcmp(e.key.data(), h, bytes) == 0) {
+            *dev = e.ptr;
+            return SSAM_OK;
+        }
+    if (v.size() >= kMaxTapCopies) {
+        if (hipDeviceSynchronize() != hipSuccess) return SSAM_E_HIP;
+        for (auto& e : v) (void)hipFree(e.ptr);
+        v.clear();
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return SSAM_E_HIP;
+    if (hipMemcpy(p, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return SSAM_E_HIP;
+    }
+    v.push_back(TapCopy{std::vector<unsigned char>(h, h + bytes), p});
+    *dev = p;
     return SSAM_OK;
 }
 
@@ -289,8 +366,22 @@ int conv_call(int dtype, const T* src, T* dst, int width, int width_stride, int 
               const T* weight, int fw, int fh, void* stream) {
     if (!weight || fw < 1 || fh < 1) return SSAM_E_INVALID;
     if (fw > 2 * kMaxBoxRadius + 1 || fh > 2 * kMaxBoxRadius + 1) return SSAM_E_UNSUPPORTED;
-    std::vector<T> hw(static_cast<size_t>(fw) * fh);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (fw == fh) {
+        // square filters (every size the reference harness runs, conv2D.cu:186-212):
+        // the kernel reads the device weights in place -- no host round trip, so the
+        // call stays asynchronous and graph-capturable.  Even sizes fill rows and
+        // columns 0..fw-1 of a (fw+1)^2 box whose last row and column are zero.
+        // (a 1x1 filter sits at the centre of the smallest, 3x3, box: host embedding)
+        if (fw >= 2) {
+            const TapSrc ts{nullptr, weight};
+            return stencil2d_run(dtype, fw % 2 ? kBox : kBoxEven, fw / 2, 1, SSAM_BOUNDARY_ZERO, ts,
+                                 src, dst, width, height, width_stride, nullptr, st);
+        }
+    }
+    // non-square and 1x1 filters: read the weights back and embed them on the host;
+    // this synchronises `stream`
+    std::vector<T> hw(static_cast<size_t>(fw) * fh);
     if (hipMemcpyAsync(hw.data(), weight, hw.size() * sizeof(T), hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -594,11 +685,12 @@ int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst
     // validate once through the normal path (also warms the code object)
     int rc = stencil2d_impl(d, src, dst, width, height, pitch, work, st);
     if (rc != SSAM_OK) return rc;
-    int R = d->fw / 2 > d->fh / 2 ? d->fw / 2 : d->fh / 2;
-    if (R < 1) R = 1;
-    const int kshape = d->shape == SSAM_SHAPE_STAR
-                           ? kStar
-                           : ((d->flags & SSAM_FLAG_COMPENSATED) ? kBoxKahan : kBox);
+    int kshape = 0, R = 0;
+    {
+        std::vector<double> taps;
+        rc = kernel_form(d, &kshape, &R, &taps);
+        if (rc != SSAM_OK) return rc;
+    }
     const int fmax = max_fused(d->storage, d->shape, R);
     const int wps_c[] = {2, 4, 8, 12, 24};
     std::vector<int> tch_c;

@@ -219,6 +219,7 @@ void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p,
     p->nstrips = static_cast<int>((it.nx + I::G::OW - 1) / I::G::OW);
     p->seg_rows = seg_rows;
     p->boundary = kZero;
+    p->taps = nullptr;  // the item's taps reach the tile as a pointer into the table
     p->vec_ok = (reinterpret_cast<uintptr_t>(it.src) % sizeof(S) == 0) &&
                 (reinterpret_cast<uintptr_t>(it.dst) % sizeof(S) == 0);
     const int nsegs = (p->height + seg_rows - 1) / seg_rows;

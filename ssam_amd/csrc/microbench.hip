@@ -30,7 +30,15 @@ enum MicroOp : int {
     kMbBpermTput = 6,
     kMbLdsTput = 7,
     kMbMfmaTput = 8,
-    kMbCount = 9
+    // device rates in TFLOP/s from HIP events around a long kernel (wall clock, no
+    // cycle counter): FP32 FMA (v_fmac_f32, no packed FP32), FP64 FMA, and the FP32 /
+    // FP64 matrix instructions the banded-GEMM forms use
+    kMbFmaF32Rate = 9,
+    kMbFmaF64Rate = 10,
+    kMbMfmaF32x4Rate = 11,   // v_mfma_f32_16x16x4_f32
+    kMbMfmaF32x1Rate = 12,   // v_mfma_f32_4x4x1_16b_f32
+    kMbMfmaF64Rate = 13,     // v_mfma_f64_16x16x4_f64
+    kMbCount = 14
 };
 
 constexpr int kMbIters = 1024;  // dependent operations per chain
@@ -114,6 +122,53 @@ __global__ __launch_bounds__(256) void mb_tput_kernel(float a, unsigned long lon
     sink[blockIdx.x * 256 + threadIdx.x] = s;
 }
 
+typedef double mb_f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kMbRateIters = 4096;  // loop trips of the rate kernels (8 chains each)
+
+// Each lane runs 8 independent chains; `flops` per chain step is fixed by OP.
+template <int OP>
+__global__ __launch_bounds__(256) void mb_rate_kernel(float a, float* sink) {
+    const int lane = threadIdx.x;
+    const float fa = a * (1.0f + static_cast<float>(lane) * 1e-7f);
+    float xf[kMbChains];
+    double xd[kMbChains];
+    mb_f32x4 af[kMbChains];
+    mb_f64x4 ad[kMbChains];
+#pragma unroll
+    for (int c = 0; c < kMbChains; ++c) {
+        xf[c] = static_cast<float>(lane + c) * 1e-3f;
+        xd[c] = static_cast<double>(xf[c]);
+        af[c] = mb_f32x4{0.f, 0.f, 0.f, 0.f};
+        ad[c] = mb_f64x4{0.0, 0.0, 0.0, 0.0};
+    }
+    const double da = static_cast<double>(fa);
+    for (int i = 0; i < kMbRateIters; ++i) {
+#pragma unroll
+        for (int c = 0; c < kMbChains; ++c) {
+            if constexpr (OP == kMbFmaF32Rate) xf[c] = __builtin_fmaf(xf[c], fa, 1.0f);
+            else if constexpr (OP == kMbFmaF64Rate) xd[c] = __builtin_fma(xd[c], da, 1.0);
+            else if constexpr (OP == kMbMfmaF32x4Rate)
+                af[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, xf[c], af[c], 0, 0, 0);
+            else if constexpr (OP == kMbMfmaF32x1Rate)
+                af[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(fa, xf[c], af[c], 0, 0, 0);
+            else
+                ad[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(da, xd[c], ad[c], 0, 0, 0);
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < kMbChains; ++c)
+        s += xf[c] + static_cast<float>(xd[c]) + af[c][0] + af[c][3] + static_cast<float>(ad[c][0] + ad[c][3]);
+    sink[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// flops of one chain step of one wave
+template <int OP> constexpr double mb_wave_flops() {
+    return OP == kMbFmaF32Rate || OP == kMbFmaF64Rate ? 2.0 * 64
+           : OP == kMbMfmaF32x1Rate                   ? 2.0 * 16 * 4 * 4 * 1
+                                                      : 2.0 * 16 * 16 * 4;
+}
+
 }  // namespace ssam
 
 using namespace ssam;
@@ -176,6 +231,36 @@ template <int OP> int run_tput(double* result, hipStream_t st) {
     return rc;
 }
 
+template <int OP> int run_rate(double* result, hipStream_t st) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return SSAM_E_HIP;
+    const int blocks = 8 * cus;  // 32 waves per CU
+    float* d_sink = nullptr;
+    if (hipMalloc(&d_sink, static_cast<size_t>(blocks) * 256 * sizeof(float)) != hipSuccess) return SSAM_E_HIP;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SSAM_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SSAM_E_HIP;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && rc == SSAM_OK; ++rep) {  // first run warms the clocks
+        float ms = 0.f;
+        if (hipEventRecord(e0, st) != hipSuccess) { rc = SSAM_E_HIP; break; }
+        hipLaunchKernelGGL(mb_rate_kernel<OP>, dim3(blocks), dim3(256), 0, st, 1.0001f, d_sink);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = SSAM_E_HIP;
+        else if (rep > 0 && ms < best)
+            best = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_sink);
+    const double flops = static_cast<double>(blocks) * 4 * kMbRateIters * kMbChains * mb_wave_flops<OP>();
+    *result = rc == SSAM_OK ? flops / (static_cast<double>(best) * 1e-3) / 1e12 : 0.0;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" int ssam_microbench(int which, double* result, void* stream) {
@@ -190,6 +275,11 @@ extern "C" int ssam_microbench(int which, double* result, void* stream) {
         case kMbDppTput: return run_tput<kMbDppTput>(result, st);
         case kMbBpermTput: return run_tput<kMbBpermTput>(result, st);
         case kMbLdsTput: return run_tput<kMbLdsTput>(result, st);
-        default: return run_tput<kMbMfmaTput>(result, st);
+        case kMbMfmaTput: return run_tput<kMbMfmaTput>(result, st);
+        case kMbFmaF32Rate: return run_rate<kMbFmaF32Rate>(result, st);
+        case kMbFmaF64Rate: return run_rate<kMbFmaF64Rate>(result, st);
+        case kMbMfmaF32x4Rate: return run_rate<kMbMfmaF32x4Rate>(result, st);
+        case kMbMfmaF32x1Rate: return run_rate<kMbMfmaF32x1Rate>(result, st);
+        default: return run_rate<kMbMfmaF64Rate>(result, st);
     }
 }

@@ -10,7 +10,9 @@ namespace ssam {
 // 2D kernel shapes.  kBoxKahan: box / dense taps with compensated accumulation
 // (each filter row is summed on its own, then added to the running sum with
 // Kahan compensation).
-enum Shape2D : int { kStar = 0, kBox = 1, kBoxKahan = 2 };
+// kBoxEven: an even square filter (2r x 2r taps, stored densely) in the
+// (2r+1)^2 box, whose last row and column are zero.
+enum Shape2D : int { kStar = 0, kBox = 1, kBoxKahan = 2, kBoxEven = 3 };
 
 // A validated 2D request: one launch of `steps` fused sweeps (steps in 1,2,4,8).
 struct Req2D {
@@ -21,7 +23,11 @@ struct Req2D {
     int radius;      // filter half-width after embedding
     int steps;       // fused steps of this launch
     int boundary;    // Boundary
-    const double* taps;  // (2r+1)^2 row-major, host
+    const double* taps;  // host taps, row-major: (2r+1)^2, or (2r)^2 for kBoxEven;
+                         // nullptr when dev_taps is set
+    // Device taps in the accumulation type, read in place by the kernel (the
+    // reference conv2D's device weights), laid out as the shape's host taps.
+    const void* dev_taps;
     int seg_rows;        // 0 = heuristic
     int waves_per_simd;  // tile-count target for the heuristic
     int form;            // Form2D: kernel family (auto picks the matrix cores where they apply)
@@ -31,6 +37,12 @@ struct Req2D {
 enum Form2D : int { kFormAuto = 0, kFormValu = 1, kFormMfma = 2 };
 
 template <typename S> int dispatch2d(const Req2D& rq, hipStream_t st);
+
+// Taps too large for the kernel-argument segment (FP64 filters past 22 x 22
+// given as host coefficients): a device copy in the accumulation type, kept for
+// the life of the process and keyed by its contents, so only the first call with
+// a given filter uploads (synchronously); SSAM_OK or an error status.
+int device_tap_copy(const void* host, size_t bytes, const void** dev);
 
 // A validated 3D request: one sweep.
 struct Req3D {

@@ -48,6 +48,22 @@ struct Params2D {
     int vec_ok;       // strips may use one vector access (pitch, pointers aligned)
     int nblocks;      // gridDim.x (for the XCD remap)
     int boundary;     // Boundary rule (kZero, kClamp, kHold)
+    const void* taps;  // device taps (nullptr: the kernel-argument segment)
+};
+
+// The taps of one box as the kernel reads them: scalar loads at uniform,
+// compile-time offsets from one base (the kernel-argument segment, a batch table
+// or the caller's device array).  kBoxEven: an even square filter of 2r x 2r taps
+// stored densely (pitch 2r) at the top-left of the (2r+1)^2 box, whose last row
+// and column are zero and never read (the reference correlation's offset n - fw/2,
+// ssai-2dconv/common.h:196-197).
+template <typename A, int K, int SHAPE> struct TapView {
+    using CP = const __attribute__((address_space(4))) A*;  // constant space: scalar loads
+    CP w;
+    static constexpr bool kSkip(int n, int c) { return SHAPE == kBoxEven && (n == K - 1 || c == K - 1); }
+    __device__ __forceinline__ A at(int n, int c) const {
+        return SHAPE == kBoxEven ? w[n * (K - 1) + c] : w[n * K + c];
+    }
 };
 
 constexpr int kWavesPerBlock2D = 4;
@@ -82,8 +98,8 @@ template <int R, int V, int T> struct Geom2D {
 // travel systolically: a = P[DR]; a = shl(a) + P[DR-1]; ...; out += shl(a)
 // (wave_shl:1, fused into v_add_f32_dpp), and the mirror image with wave_shr:1.
 template <typename A, int R, int SHAPE, int V>
-__device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const A* __restrict__ w,
-                                            A (&out)[V]) {
+__device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V],
+                                            const TapView<A, 2 * R + 1, SHAPE>& w, A (&out)[V]) {
     constexpr int K = 2 * R + 1;
     constexpr int D = (R + V - 1) / V;  // largest lane distance
     constexpr bool KAHAN = SHAPE == kBoxKahan;
@@ -104,7 +120,8 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const A*
 #pragma unroll
         for (int m = -R; m <= R; ++m) {
             if (SHAPE == kStar && n != R && m != 0) continue;
-            const A wt = w[n * K + m + R];
+            if (TapView<A, K, SHAPE>::kSkip(n, m + R)) continue;
+            const A wt = w.at(n, m + R);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int c = j + m;
@@ -277,7 +294,7 @@ struct Stencil2D {
         // into (uniform) VGPRs, which large boxes cannot afford
         int zero = 0;
         asm volatile("" : "+s"(zero));
-        const A* __restrict__ tw = w + zero;
+        const TapView<A, K, SHAPE> tw{(typename TapView<A, K, SHAPE>::CP)(w) + zero};
         if constexpr (Row::kPacked) {
 #pragma unroll
             for (int j = 0; j < V; ++j) cv[s0(KT, 2 * R)][j] = ring0[s0(KT, 2 * R)].get(j);
@@ -435,14 +452,24 @@ struct Stencil2D {
 // The taps are the FIRST kernel argument, so they sit at offset 0 of the kernarg
 // segment; the kernel reads them there with scalar loads instead of copying the
 // by-value struct into private memory.
+// Taps passed by value must leave room in the 4 KiB kernel-argument segment;
+// larger sets (FP64 filters past 22 x 22) always come from device memory.
+constexpr int kMaxKernargTapBytes = 3584;
+template <typename A, int R, int SHAPE> struct KernargTaps {
+    static constexpr int n = SHAPE == kBoxEven ? 4 * R * R : (2 * R + 1) * (2 * R + 1);
+    static constexpr bool fits = n * static_cast<int>(sizeof(A)) <= kMaxKernargTapBytes;
+    static constexpr int count = fits ? n : 1;
+};
+
 template <typename S, int R, int SHAPE, int V, int T, int PF, bool RING, int U, int WPE>
 __global__ __launch_bounds__(kWave * kWavesPerBlock2D) __attribute__((amdgpu_waves_per_eu(WPE)))
 void ssam2d_kernel(
-    Taps<typename Storage<S>::acc, (2 * R + 1) * (2 * R + 1)> tp, const S* __restrict__ src,
-    S* __restrict__ dst, Params2D p) {
+    Taps<typename Storage<S>::acc, KernargTaps<typename Storage<S>::acc, R, SHAPE>::count> tp,
+    const S* __restrict__ src, S* __restrict__ dst, Params2D p) {
     (void)tp;
     using A = typename Storage<S>::acc;
-    const A* __restrict__ w = (const A*)(__builtin_amdgcn_kernarg_segment_ptr());
+    const A* __restrict__ w = p.taps ? static_cast<const A*>(p.taps)
+                                     : (const A*)(__builtin_amdgcn_kernarg_segment_ptr());
     Stencil2D<S, R, SHAPE, V, T, PF, RING, U>::run(src, dst, p, w);
 }
 

@@ -1,0 +1,4 @@
+// stencil2d_box_f64.hip -- 2D SSAM kernel instances for f64 storage: full (2r+1)^2 boxes and dense filters embedded in them, radius 1..14.
+#define SSAM_2D_SHAPE_UNIT
+#include "stencil2d_dispatch.hpp"
+SSAM_INSTANTIATE_2D_SHAPE(double, ssam::kBox)

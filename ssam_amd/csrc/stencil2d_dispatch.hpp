@@ -4,6 +4,8 @@
 #pragma once
 #include "stencil2d.hpp"
 #include "stencil2d_mfma.hpp"
+#include <vector>
+
 #include "ssam_internal.hpp"
 
 namespace ssam {
@@ -42,17 +44,27 @@ template <typename S, int R, int SHAPE, int V, int T>
 static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     using A = typename Storage<S>::acc;
     using G = Geom2D<R, V, T>;
-    constexpr int K = 2 * R + 1;
     using UR = Unroll2D<R, SHAPE, V, T>;
     constexpr int PF = UR::PF;
     constexpr bool RING = UR::RING;
     constexpr int U = UR::U;
     constexpr int WPE = UR::WPE;
 
-    Taps<A, K * K> tp;
-    for (int i = 0; i < K * K; ++i) tp.w[i] = static_cast<A>(rq.taps[i]);
-
+    using KT = KernargTaps<A, R, SHAPE>;
+    Taps<A, KT::count> tp{};
     Params2D p;
+    p.taps = nullptr;
+    if (rq.dev_taps) {
+        p.taps = rq.dev_taps;
+    } else if constexpr (KT::fits) {
+        for (int i = 0; i < KT::n; ++i) tp.w[i] = static_cast<A>(rq.taps[i]);
+    } else {
+        std::vector<A> hw(KT::n);
+        for (int i = 0; i < KT::n; ++i) hw[i] = static_cast<A>(rq.taps[i]);
+        const int rc = device_tap_copy(hw.data(), hw.size() * sizeof(A), &p.taps);
+        if (rc != SSAM_OK) return rc;
+    }
+
     p.width = static_cast<int>(rq.width);
     p.height = static_cast<int>(rq.height);
     p.pitch = rq.pitch;
@@ -87,7 +99,14 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
 }
 
-// radius is a compile-time instance parameter; these expand the runtime switch
+// radius is a compile-time instance parameter; these expand the runtime switch.
+// Stars go to radius 6, compensated boxes to 9 (19 x 19), boxes and even filters
+// to 14: every filter size 2..28 the reference conv2D dispatches
+// (ssai-2dconv/conv2D.cu:186-212) embeds in a box of radius <= 14.
+template <int SHAPE> constexpr int max_radius() {
+    return SHAPE == kStar ? 6 : SHAPE == kBoxKahan ? 9 : 14;
+}
+
 template <typename S, int SHAPE, int V, int T>
 static int dispatch_radius(const Req2D& rq, hipStream_t st) {
     switch (rq.radius) {
@@ -110,8 +129,29 @@ static int dispatch_radius(const Req2D& rq, hipStream_t st) {
                 break;
         }
     }
+    if constexpr (SHAPE != kStar && SHAPE != kBoxKahan && T == 1) {
+        switch (rq.radius) {
+#define SSAM_R(r) \
+    case r:       \
+        return launch2d_instance<S, r, SHAPE, V, T>(rq, st);
+            SSAM_R(10) SSAM_R(11) SSAM_R(12) SSAM_R(13) SSAM_R(14)
+#undef SSAM_R
+            default:
+                break;
+        }
+    }
     return SSAM_E_UNSUPPORTED;
 }
+
+// One sweep of a box-family shape (kBox, kBoxKahan, kBoxEven) on the VALU kernel.
+// Each (storage, shape) has its own compile unit (stencil2d_<shape>_<type>.hip,
+// SSAM_INSTANTIATE_2D_SHAPE), so the large-radius instances build in parallel.
+template <typename S, int SHAPE> int dispatch_shape(const Req2D& rq, hipStream_t st);
+#ifdef SSAM_2D_SHAPE_UNIT
+template <typename S, int SHAPE> int dispatch_shape(const Req2D& rq, hipStream_t st) {
+    return dispatch_radius<S, SHAPE, Lanes2D<S>::V, 1>(rq, st);
+}
+#endif
 
 template <typename S>
 int dispatch2d(const Req2D& rq, hipStream_t st) {
@@ -132,8 +172,15 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
             return dispatch_radius<S, kStar, V, 1>(rq, st);
         }
         // FP64 storage accumulates in FP64: compensation changes nothing there
-        if (rq.shape == kBoxKahan && sizeof(S) != 8) return dispatch_radius<S, kBoxKahan, V, 1>(rq, st);
-        return dispatch_radius<S, kBox, V, 1>(rq, st);
+        if constexpr (sizeof(S) != 8) {
+            if (rq.shape == kBoxKahan) return dispatch_shape<S, kBoxKahan>(rq, st);
+        }
+        // even square filters (F32 / F64; 2-byte storage embeds them in a box)
+        if constexpr (sizeof(S) >= 4) {
+            if (rq.shape == kBoxEven) return dispatch_shape<S, kBoxEven>(rq, st);
+        }
+        if (rq.shape == kBoxEven) return SSAM_E_UNSUPPORTED;
+        return dispatch_shape<S, kBox>(rq, st);
     }
     // fused time steps: star radius 1 and 2 (the reference's temporal 5-point and
     // 9-point star kernels), T in {2, 4, 8}
@@ -158,3 +205,6 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
 
 #define SSAM_INSTANTIATE_2D(S) \
     template int ssam::dispatch2d<S>(const ssam::Req2D&, hipStream_t);
+
+#define SSAM_INSTANTIATE_2D_SHAPE(S, SHAPE) \
+    template int ssam::dispatch_shape<S, SHAPE>(const ssam::Req2D&, hipStream_t);

@@ -1,0 +1,4 @@
+// stencil2d_even_f32.hip -- 2D SSAM kernel instances for f32 storage: even square filters (2r x 2r taps), radius 1..14.
+#define SSAM_2D_SHAPE_UNIT
+#include "stencil2d_dispatch.hpp"
+SSAM_INSTANTIATE_2D_SHAPE(float, ssam::kBoxEven)

@@ -1,0 +1,4 @@
+// stencil2d_kahan_f16.hip -- 2D SSAM kernel instances for f16 storage: compensated boxes and dense filters, radius 1..9.
+#define SSAM_2D_SHAPE_UNIT
+#include "stencil2d_dispatch.hpp"
+SSAM_INSTANTIATE_2D_SHAPE(ssam::f16_t, ssam::kBoxKahan)

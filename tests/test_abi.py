@@ -71,11 +71,11 @@ def test_stencil2d_rejects_malformed_requests():
     assert _s2d(dd) == L.E_INVALID
     dd, _ = _desc(steps=3)
     assert _s2d(dd, work=FAKE_A) == L.E_INVALID             # work aliases src
-    dd, _ = _desc(shape=L.BOX, k=21)                        # radius 10 box
+    dd, _ = _desc(shape=L.BOX, k=31)                        # radius 15 box
     assert _s2d(dd) == L.E_UNSUPPORTED
     dd, _ = _desc(shape=L.STAR, k=15)                       # radius 7 star
     assert _s2d(dd) == L.E_UNSUPPORTED
-    dd, _ = _desc(shape=L.DENSE, coeffs=np.ones((3, 21)))
+    dd, _ = _desc(shape=L.DENSE, coeffs=np.ones((3, 31)))  # 31 columns: past 29 x 29
     assert _s2d(dd) == L.E_UNSUPPORTED
 
 
@@ -89,6 +89,9 @@ def test_stencil2d_flags():
     dd, _ = _desc(shape=L.BOX, steps=2)                     # compensated fused steps
     dd.flags = L.FLAG_COMPENSATED
     assert _s2d(dd, work=FAKE_W) == L.E_UNSUPPORTED
+    dd, _ = _desc(shape=L.BOX, k=21)                        # compensated radius 10
+    dd.flags = L.FLAG_COMPENSATED
+    assert _s2d(dd) == L.E_UNSUPPORTED
 
 
 def test_reference_entries_validate():
@@ -96,10 +99,12 @@ def test_reference_entries_validate():
     assert lib.ssam_j2dstar_f32(FAKE_A, FAKE_B, 64, 64, 1, None, None) == L.E_INVALID
     c = (ctypes.c_float * 33)()
     assert lib.ssam_j2dstar_f32(FAKE_A, FAKE_B, 64, 64, 8, c, None) == L.E_UNSUPPORTED
-    assert lib.ssam_j2dbox_f64(FAKE_A, FAKE_B, 64, 64, 10, c, None) == L.E_UNSUPPORTED
+    assert lib.ssam_j2dbox_f64(FAKE_A, FAKE_B, 64, 64, 15, c, None) == L.E_UNSUPPORTED
     assert lib.ssam_j2d5pt_f32(FAKE_A, FAKE_A, 64, 64, 1, 1, 1, 1, 1, None) == L.E_INVALID
     assert lib.ssam_conv2d_f32(FAKE_A, FAKE_B, 64, 64, 64, None, 3, 3, None) == L.E_INVALID
-    assert lib.ssam_conv2d_f32(FAKE_A, FAKE_B, 64, 64, 64, FAKE_W, 21, 3, None) == L.E_UNSUPPORTED
+    # the reference dispatches filter sizes 2..28 (conv2D.cu:186-212); 30 is past that
+    assert lib.ssam_conv2d_f32(FAKE_A, FAKE_B, 64, 64, 64, FAKE_W, 30, 30, None) == L.E_UNSUPPORTED
+    assert lib.ssam_conv2d_f64(FAKE_A, FAKE_B, 64, 64, 64, FAKE_W, 31, 3, None) == L.E_UNSUPPORTED
 
 
 def test_stencil3d_validates():

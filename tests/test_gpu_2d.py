@@ -77,7 +77,10 @@ def test_kat_star_integer_exact(golden_dir, r, dt):
     assert np.array_equal(got, d["out"])
 
 
-@pytest.mark.parametrize("k", [2, 3, 5, 7, 9, 15, 19])
+CONV_K = [2, 3, 5, 7, 9, 15, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28]
+
+
+@pytest.mark.parametrize("k", CONV_K)
 @pytest.mark.parametrize("dt", ["float32", "float64"])
 def test_kat_conv_integer_exact(golden_dir, k, dt):
     d = np.load(golden_dir / f"kat_conv{k}_int_{dt}.npz")
@@ -89,6 +92,39 @@ def test_kat_conv_integer_exact(golden_dir, k, dt):
     api.conv2d(src, dst, torch.from_numpy(d["filt"]).cuda())
     torch.cuda.synchronize()
     assert np.array_equal(dst.cpu().numpy(), d["out"])
+
+
+@pytest.mark.parametrize("k", [4, 19, 28])
+def test_conv_entry_is_graph_capturable(golden_dir, k):
+    # square filters read the device weights in place: no host round trip, so the
+    # reference-shaped entry records into a graph; every replay must equal the
+    # eager result (with eager work and synchronisation between replays)
+    d = np.load(golden_dir / f"kat_conv{k}_int_float32.npz") if k != 4 else None
+    if d is None:
+        x = np.floor(oracle.uniform(31, 53 * 71, 0.0, 256.0, np.float32)).reshape(53, 71)
+        f = np.floor(oracle.uniform(32, 16, -10.0, 11.0, np.float32)).reshape(4, 4)
+        want = oracle.conv2d(x, f)
+    else:
+        x, f, want = d["src"], d["filt"], d["out"]
+    src = dev(x, pitch_pad=3)
+    dst = dev(np.zeros_like(x), pitch_pad=3)
+    wt = torch.from_numpy(np.ascontiguousarray(f)).cuda()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        api.conv2d(src, dst, wt, stream=s)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    assert np.array_equal(dst.cpu().numpy(), want)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        api.conv2d(src, dst, wt)
+    for rep in range(3):
+        dst.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(dst.cpu().numpy(), want), f"replay {rep}"
 
 
 @pytest.mark.parametrize("name,r", [("box64", 4), ("box81", 4), ("box121", 5)])

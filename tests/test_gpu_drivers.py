@@ -73,11 +73,14 @@ def test_3dstencil_driver(rundir, fn, dtype):
     assert rms <= 1e-5, out
 
 
-@pytest.mark.parametrize("fsize", [3, 5, 9, 19])
-def test_2dconv_driver_and_csv(rundir, fsize):
-    out = _run(rundir, "ssai-2dconv", "float", 96, fsize)
+# the reference's run.sh loops fsize 2..21 and conv2D.cu:186-212 dispatches up to 28
+@pytest.mark.parametrize("fsize", [2, 3, 5, 9, 19, 20, 21, 28])
+@pytest.mark.parametrize("dtype", ["float", "double"])
+def test_2dconv_driver_and_csv(rundir, fsize, dtype):
+    out = _run(rundir, "ssai-2dconv", dtype, 96, fsize)
     assert "Load success : ../data/Lena96x96.raw" in out
-    assert float(re.search(r"verify dif =([-+0-9.eE]+)", out).group(1)) <= 1e-2
+    # 8-bit image and integer taps in [-10, 10]: every sum is exact (255*10*28^2 < 2^24)
+    assert float(re.search(r"verify dif =([-+0-9.eE]+)", out).group(1)) == 0.0, out
     rows = (rundir / "log.conv2D.csv").read_text().strip().splitlines()
     assert re.fullmatch(rf"96x96, 0_{fsize}, 256, {fsize}x{fsize}, [0-9.]+", rows[-1]), rows[-1]
 

@@ -51,7 +51,8 @@ def test_conv_matches_reference_gold_fixtures(golden_dir):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("fh,fw", [(3, 3), (2, 2), (5, 3), (4, 7), (9, 9), (1, 1)])
+@pytest.mark.parametrize("fh,fw", [(3, 3), (2, 2), (5, 3), (4, 7), (9, 9), (1, 1), (20, 20), (23, 23),
+                                   (28, 28)])
 def test_conv_matches_reference_live(dtype, fh, fw):
     if oracle.ref_lib() is None:
         pytest.skip("oracle/_ref (the reference gold) not built here")
