@@ -38,7 +38,12 @@ enum MicroOp : int {
     kMbMfmaF32x4Rate = 11,   // v_mfma_f32_16x16x4_f32
     kMbMfmaF32x1Rate = 12,   // v_mfma_f32_4x4x1_16b_f32
     kMbMfmaF64Rate = 13,     // v_mfma_f64_16x16x4_f64
-    kMbCount = 14
+    // FP32 FMA operand forms of the stencil inner loop (TFLOP/s, wall clock):
+    kMbFmacVgprRate = 14,    // v_fmac_f32 acc, v_tap, v_x (every operand a VGPR)
+    kMbFmacSgprRate = 15,    // v_fmac_f32 acc, s_tap, v_x (one uniform tap)
+    kMbFmacTapsRate = 16,    // v_fmac_f32 acc, s_tap[k], v_x over 16 kernel-argument taps
+    kMbFmacDppRate = 17,     // 4 FMAs per DPP-fused add (the systolic hop's share)
+    kMbCount = 18
 };
 
 constexpr int kMbIters = 1024;  // dependent operations per chain
@@ -162,9 +167,40 @@ __global__ __launch_bounds__(256) void mb_rate_kernel(float a, float* sink) {
     sink[blockIdx.x * 256 + threadIdx.x] = s;
 }
 
+// The stencil's FMA forms: 8 accumulators per lane, data x[c] fixed per lane; the
+// taps come from VGPRs, one SGPR, or 16 kernel-argument taps read as SGPRs.
+struct MbTaps { float t[16]; };
+template <int OP>
+__global__ __launch_bounds__(256) void mb_fmac_kernel(MbTaps tp, float* sink) {
+    const int lane = threadIdx.x;
+    float x[kMbChains], acc[kMbChains];
+    const float tv = tp.t[0] * (1.0f + static_cast<float>(lane) * 1e-7f);  // lane-varying tap
+#pragma unroll
+    for (int c = 0; c < kMbChains; ++c) {
+        x[c] = static_cast<float>(lane + c) * 1e-3f;
+        acc[c] = 0.f;
+    }
+    for (int i = 0; i < kMbRateIters; ++i) {
+#pragma unroll
+        for (int c = 0; c < kMbChains; ++c) {
+            if constexpr (OP == kMbFmacVgprRate) acc[c] = __builtin_fmaf(tv, x[c], acc[c]);
+            else if constexpr (OP == kMbFmacSgprRate) acc[c] = __builtin_fmaf(tp.t[0], x[c], acc[c]);
+            else if constexpr (OP == kMbFmacTapsRate) acc[c] = __builtin_fmaf(tp.t[(c + 8 * (i & 1)) & 15], x[c], acc[c]);
+            else {
+                acc[c] = __builtin_fmaf(tp.t[c], x[c], acc[c]);
+                if ((c & 3) == 3) acc[c] = from_left(acc[c - 3]) + acc[c];
+            }
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < kMbChains; ++c) s += acc[c];
+    sink[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 // flops of one chain step of one wave
 template <int OP> constexpr double mb_wave_flops() {
-    return OP == kMbFmaF32Rate || OP == kMbFmaF64Rate ? 2.0 * 64
+    return OP == kMbFmaF32Rate || OP == kMbFmaF64Rate || OP >= kMbFmacVgprRate ? 2.0 * 64
            : OP == kMbMfmaF32x1Rate                   ? 2.0 * 16 * 4 * 4 * 1
                                                       : 2.0 * 16 * 16 * 4;
 }
@@ -246,7 +282,13 @@ template <int OP> int run_rate(double* result, hipStream_t st) {
     for (int rep = 0; rep < 3 && rc == SSAM_OK; ++rep) {  // first run warms the clocks
         float ms = 0.f;
         if (hipEventRecord(e0, st) != hipSuccess) { rc = SSAM_E_HIP; break; }
-        hipLaunchKernelGGL(mb_rate_kernel<OP>, dim3(blocks), dim3(256), 0, st, 1.0001f, d_sink);
+        if constexpr (OP >= kMbFmacVgprRate) {
+            MbTaps tp;
+            for (int k = 0; k < 16; ++k) tp.t[k] = 1.0001f + 1e-4f * k;
+            hipLaunchKernelGGL(mb_fmac_kernel<OP>, dim3(blocks), dim3(256), 0, st, tp, d_sink);
+        } else {
+            hipLaunchKernelGGL(mb_rate_kernel<OP>, dim3(blocks), dim3(256), 0, st, 1.0001f, d_sink);
+        }
         if (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
             hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
             rc = SSAM_E_HIP;
@@ -280,6 +322,10 @@ extern "C" int ssam_microbench(int which, double* result, void* stream) {
         case kMbFmaF64Rate: return run_rate<kMbFmaF64Rate>(result, st);
         case kMbMfmaF32x4Rate: return run_rate<kMbMfmaF32x4Rate>(result, st);
         case kMbMfmaF32x1Rate: return run_rate<kMbMfmaF32x1Rate>(result, st);
-        default: return run_rate<kMbMfmaF64Rate>(result, st);
+        case kMbMfmaF64Rate: return run_rate<kMbMfmaF64Rate>(result, st);
+        case kMbFmacVgprRate: return run_rate<kMbFmacVgprRate>(result, st);
+        case kMbFmacSgprRate: return run_rate<kMbFmacSgprRate>(result, st);
+        case kMbFmacTapsRate: return run_rate<kMbFmacTapsRate>(result, st);
+        default: return run_rate<kMbFmacDppRate>(result, st);
     }
 }

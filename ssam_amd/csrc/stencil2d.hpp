@@ -60,10 +60,68 @@ struct Params2D {
 template <typename A, int K, int SHAPE> struct TapView {
     using CP = const __attribute__((address_space(4))) A*;  // constant space: scalar loads
     CP w;
+    static constexpr bool kRowPipe = false;  // taps loaded a filter row ahead
+    static constexpr bool kVgpr = false;     // FMAs read the tap SGPRs
     static constexpr bool kSkip(int n, int c) { return SHAPE == kBoxEven && (n == K - 1 || c == K - 1); }
     __device__ __forceinline__ A at(int n, int c) const {
         return SHAPE == kBoxEven ? w[n * (K - 1) + c] : w[n * K + c];
     }
+};
+
+// Where an FP32 FMA takes its tap from.  On gfx950 v_fmac_f32 issues at twice
+// the rate when every operand is a VGPR; an SGPR operand (a uniform tap) halves
+// it (measured: 119 vs 73 TFLOP/s, ssam_microbench 14 / 15).  So FP32
+// accumulation copies the taps into VGPRs: resident for the whole tile when the
+// shape has at most kMaxResidentTaps of them, else once per tick (one v_mov per
+// tap feeds its V FMAs) up to kMaxTickTaps; past that the register allocator
+// keeps too many of the copies live and spills, so the largest filters and all
+// FP64 FMAs read the tap SGPRs directly.  The per-tick VGPR form runs the
+// filter one row at a time (stencil_row): row n+1's scalar loads are issued
+// while row n computes, and a scheduling barrier between rows keeps the
+// scheduler from hoisting every copy of a large filter at once (which spilled).
+// The SGPR form keeps the scheduler's own order: barriers there exposed each
+// row's load latency (FP64 box121 ran 2.3x slower).
+enum TapMode : int { kTapSgpr = 0, kTapResident = 1, kTapTick = 2 };
+constexpr int kMaxResidentTaps = 128;
+constexpr int kMaxTickTaps = 196;
+template <typename A, int R, int SHAPE> struct TapPolicy {
+    static constexpr int K = 2 * R + 1;
+    static constexpr int used = SHAPE == kStar ? 4 * R + 1 : SHAPE == kBoxEven ? 4 * R * R : K * K;
+    // compensated boxes hold two more partial arrays; FP64 taps take two VGPRs each
+    // and only small sets stay resident (no rate to gain, but no scalar loads)
+    static constexpr int resident_max = sizeof(A) == 8 ? 32 : SHAPE == kBoxKahan ? 49 : kMaxResidentTaps;
+    static constexpr int mode = used <= resident_max                     ? kTapResident
+                                : sizeof(A) == 8 || used > kMaxTickTaps ? kTapSgpr
+                                                                         : kTapTick;
+};
+
+// a uniform value as a VGPR operand (the asm output is opaque, so the FMAs that
+// use it cannot fold the SGPR back in)
+__device__ __forceinline__ float to_vgpr(float s) {
+    float v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+__device__ __forceinline__ double to_vgpr(double s) {
+    double v;
+    asm("v_mov_b64 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+
+// per-tick VGPR copies of the taps (kTapTick)
+template <typename A, int K, int SHAPE> struct TapViewV : TapView<A, K, SHAPE> {
+    static constexpr bool kRowPipe = true;
+    static constexpr bool kVgpr = true;
+};
+
+// the taps held in VGPRs for a whole tile (kTapResident); entries of skipped
+// taps are never read and never materialised
+template <typename A, int K, int SHAPE> struct TapRegs {
+    static constexpr bool kRowPipe = false;
+    static constexpr bool kVgpr = false;
+    A t[K * K];
+    static constexpr bool kSkip(int n, int c) { return TapView<A, K, SHAPE>::kSkip(n, c); }
+    __device__ __forceinline__ A at(int n, int c) const { return t[n * K + c]; }
 };
 
 constexpr int kWavesPerBlock2D = 4;
@@ -97,9 +155,8 @@ template <int R, int V, int T> struct Geom2D {
 // briefly (large boxes do not run out of scalar registers).  The partials then
 // travel systolically: a = P[DR]; a = shl(a) + P[DR-1]; ...; out += shl(a)
 // (wave_shl:1, fused into v_add_f32_dpp), and the mirror image with wave_shr:1.
-template <typename A, int R, int SHAPE, int V>
-__device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V],
-                                            const TapView<A, 2 * R + 1, SHAPE>& w, A (&out)[V]) {
+template <typename A, int R, int SHAPE, int V, typename TW>
+__device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const TW& w, A (&out)[V]) {
     constexpr int K = 2 * R + 1;
     constexpr int D = (R + V - 1) / V;  // largest lane distance
     constexpr bool KAHAN = SHAPE == kBoxKahan;
@@ -109,8 +166,43 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V],
     for (int j = 0; j < V; ++j)
 #pragma unroll
         for (int d = 0; d < 2 * D + 1; ++d) P[j][d] = C[j][d] = 0;
+    // taps of the present filter row (row-pipelined forms; a star's rows other than
+    // the centre row hold only the centre tap)
+    auto row_taps = [&](int n, A (&t)[K]) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            if (SHAPE == kStar && n != R && c != R) continue;
+            if (TW::kSkip(n, c)) continue;
+            t[c] = w.at(n, c);
+        }
+    };
+    A rt[K];
+    if constexpr (TW::kRowPipe) row_taps(0, rt);
 #pragma unroll
     for (int n = 0; n < K; ++n) {
+        A nt[K];
+        // VGPR copies of this row's taps come first (they wait for its loads), then
+        // a barrier, so the next row's loads, issued below with this row's FMAs,
+        // are not caught by that wait
+        if constexpr (TW::kRowPipe && TW::kVgpr) {
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (SHAPE == kStar && n != R && c != R) continue;
+                if (TW::kSkip(n, c)) continue;
+                rt[c] = to_vgpr(rt[c]);
+            }
+        }
+        bool issued = false;
+        auto issue_next = [&]() {
+            if constexpr (TW::kRowPipe) {
+                if (!issued) {
+                    issued = true;
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (n + 1 < K) row_taps(n + 1, nt);
+                }
+            }
+        };
+        if constexpr (TW::kVgpr) issue_next();
         A Q[V][2 * D + 1];  // this filter row's partials (KAHAN, rows after the first)
         const bool fresh = KAHAN && n > 0;
 #pragma unroll
@@ -120,8 +212,8 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V],
 #pragma unroll
         for (int m = -R; m <= R; ++m) {
             if (SHAPE == kStar && n != R && m != 0) continue;
-            if (TapView<A, K, SHAPE>::kSkip(n, m + R)) continue;
-            const A wt = w.at(n, m + R);
+            if (TW::kSkip(n, m + R)) continue;
+            const A wt = TW::kRowPipe ? rt[m + R] : w.at(n, m + R);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int c = j + m;
@@ -131,6 +223,9 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V],
                 else
                     P[j][d + D] = fmaa(wt, X[n][c - d * V], P[j][d + D]);
             }
+            // SGPR taps: the first tap's FMAs wait for this row's loads; the next
+            // row's loads go out after them
+            issue_next();
         }
         if (fresh) {
 #pragma unroll
@@ -146,6 +241,12 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V],
                     C[j][d] = (t - P[j][d]) - y;
                     P[j][d] = t;
                 }
+        }
+        if constexpr (TW::kRowPipe) {
+            // one filter row per scheduling region (a star's one-tap rows need none)
+            if (SHAPE != kStar || n == R || n == R - 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int c = 0; c < K; ++c) rt[c] = nt[c];
         }
     }
     if (KAHAN) {
@@ -185,6 +286,7 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V],
 // is the first consumed row; the rows of an intermediate step t produced at tick
 // k live in slot (k + 2R) mod NRS.  Tick k loads row c0+k+PF and reads rows
 // c0+k-2R .. c0+k (slots k .. k+2R).
+
 //  * RING: the loop body is one full ring period (U = lcm(NR0, NRS) ticks), so
 //    every slot index is a compile-time register and nothing ever moves; loads
 //    stay in flight across ticks and blocks.
@@ -262,6 +364,26 @@ struct Stencil2D {
         }
     }
 
+    // Interior tiles store without a branch: one raw buffer store over the output
+    // row, whose offset is out of range (the hardware drops the store) for halo
+    // lanes and for ticks whose row is outside the segment.  Branches around the
+    // stores split the unrolled tick into separate scheduling regions (no
+    // interleaving of the steps of a tick, no DPP-into-add fusion across them).
+    __device__ static __forceinline__ void store_row_nb(const Lane& L, int y, const A (&v)[V]) {
+        const bool ok = L.store_lane && y >= L.ys && y < L.ye;
+        const int yy = y < 0 ? 0 : y;
+        char* row = reinterpret_cast<char*>(L.dst) +
+                    static_cast<long long>(yy) * L.p.pitch * static_cast<long long>(sizeof(S));
+        S tmp[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) tmp[j] = Storage<S>::store(v[j]);
+        constexpr int W = V * static_cast<int>(sizeof(S)) / 4;
+        uint32_t words[W];
+        __builtin_memcpy(words, tmp, sizeof(tmp));
+        buf_store<uint32_t, W>(buf_rsrc(row, static_cast<uint32_t>(L.p.width) * sizeof(S)),
+                               ok ? L.xoff : kBufOOB, words);
+    }
+
     template <bool CHK>
     __device__ static __forceinline__ void store_row(const Lane& L, int y, const A (&v)[V]) {
         if (!L.store_lane) return;
@@ -281,20 +403,25 @@ struct Stencil2D {
     // BF16 / F16: cv holds the window rows converted to FP32, each row converted
     // once, at the tick where it enters the window (the packed ring slot keeps the
     // load's result until then); unused for F32 / F64.
+    using TP = TapPolicy<A, R, SHAPE>;
+    using TRegs = TapRegs<A, K, SHAPE>;
+
     template <bool CHK, int KT>
-    __device__ static __forceinline__ void tick(const Lane& L, const A* __restrict__ w, int yc0,
-                                                Row (&ring0)[NR0], A (&cv)[NR0][V],
+    __device__ static __forceinline__ void tick(const Lane& L, const A* __restrict__ w, const TRegs& tr,
+                                                int yc0, Row (&ring0)[NR0], A (&cv)[NR0][V],
                                                 A (&rings)[NSTAGE_RINGS][NRS][V]) {
         const int yc = yc0 + KT;
         // rows past the tile's last needed row only feed discarded ticks: re-read
         // that row instead (an L2 hit) so no HBM bytes are fetched for them
         if constexpr (RING) load_row<CHK>(L, min(yc + PF, L.ylast), ring0[s0(KT, PF + 2 * R)]);
-        // taps come from the kernel-argument segment through scalar loads; the
-        // opaque zero offset keeps them from being hoisted out of the row loop
-        // into (uniform) VGPRs, which large boxes cannot afford
+        // non-resident taps come through scalar loads every tick; the opaque zero
+        // offset keeps them from being hoisted out of the row loop, which large
+        // boxes cannot afford
         int zero = 0;
         asm volatile("" : "+s"(zero));
-        const TapView<A, K, SHAPE> tw{(typename TapView<A, K, SHAPE>::CP)(w) + zero};
+        using CP = typename TapView<A, K, SHAPE>::CP;
+        const TapView<A, K, SHAPE> tw{(CP)(w) + zero};
+        const TapViewV<A, K, SHAPE> twv{{(CP)(w) + zero}};
         if constexpr (Row::kPacked) {
 #pragma unroll
             for (int j = 0; j < V; ++j) cv[s0(KT, 2 * R)][j] = ring0[s0(KT, 2 * R)].get(j);
@@ -309,7 +436,9 @@ struct Stencil2D {
                     X[n][j] = (t == 1) ? (Row::kPacked ? cv[s0(KT, n)][j] : ring0[s0(KT, n)].get(j))
                                        : rings[t - 2][sS(KT, n)][j];
             A out[V];
-            stencil_row<A, R, SHAPE, V>(X, tw, out);
+            if constexpr (TP::mode == kTapResident) stencil_row<A, R, SHAPE, V>(X, tr, out);
+            else if constexpr (TP::mode == kTapTick) stencil_row<A, R, SHAPE, V>(X, twv, out);
+            else stencil_row<A, R, SHAPE, V>(X, tw, out);
             const int q = yc - t * R;  // grid row produced by this step
             if (t < T) {
                 if constexpr (CHK) {
@@ -324,6 +453,8 @@ struct Stencil2D {
                 }
 #pragma unroll
                 for (int j = 0; j < V; ++j) rings[t - 1][sS(KT, 2 * R)][j] = out[j];
+            } else if constexpr (!CHK && TP::mode != kTapSgpr) {
+                store_row_nb(L, q, out);
             } else if (q >= L.ys && q < L.ye) {
                 if constexpr (CHK && T == 1) {
                     if (L.p.boundary == kHold) {
@@ -344,12 +475,19 @@ struct Stencil2D {
     }
 
     template <bool CHK, int KT>
-    __device__ static __forceinline__ void block(const Lane& L, const A* __restrict__ w, int yc0,
-                                                 Row (&ring0)[NR0], A (&cv)[NR0][V],
+    __device__ static __forceinline__ void block(const Lane& L, const A* __restrict__ w, const TRegs& tr,
+                                                 int yc0, Row (&ring0)[NR0], A (&cv)[NR0][V],
                                                  A (&rings)[NSTAGE_RINGS][NRS][V]) {
         if constexpr (KT < U) {
-            tick<CHK, KT>(L, w, yc0, ring0, cv, rings);
-            block<CHK, KT + 1>(L, w, yc0, ring0, cv, rings);
+            tick<CHK, KT>(L, w, tr, yc0, ring0, cv, rings);
+            // Boxes with VGPR taps: one tick per scheduling region -- interleaving
+            // whole ticks (free once the interior stores stopped branching) ran
+            // past the register file.  Stars (few taps) interleave freely: a
+            // barrier here cost the BF16 stars 20%.  SGPR taps keep the scheduler's
+            // order, which hoists the next tick's tap loads (with a barrier here
+            // they spilled).
+            if constexpr (SHAPE != kStar && TP::mode != kTapSgpr) __builtin_amdgcn_sched_barrier(0);
+            block<CHK, KT + 1>(L, w, tr, yc0, ring0, cv, rings);
         }
     }
 
@@ -359,6 +497,18 @@ struct Stencil2D {
         Row ring0[NR0];
         A cv[NR0][V];
         A rings[NSTAGE_RINGS][NRS][V];
+        TRegs tr;
+        if constexpr (TP::mode == kTapResident) {
+            const TapView<A, K, SHAPE> tv{(typename TapView<A, K, SHAPE>::CP)(w)};
+#pragma unroll
+            for (int n = 0; n < K; ++n)
+#pragma unroll
+                for (int c = 0; c < K; ++c) {
+                    if (SHAPE == kStar && n != R && c != R) continue;
+                    if (TRegs::kSkip(n, c)) continue;
+                    tr.t[n * K + c] = to_vgpr(tv.at(n, c));
+                }
+        }
 #pragma unroll
         for (int s = 0; s < NSTAGE_RINGS; ++s)
 #pragma unroll
@@ -383,7 +533,7 @@ struct Stencil2D {
                 for (int k = 0; k < U; ++k)
                     load_row<CHK>(L, min(yc0 + k + PF, L.ylast), ring0[2 * R + PF + k]);
             }
-            block<CHK, 0>(L, w, yc0, ring0, cv, rings);
+            block<CHK, 0>(L, w, tr, yc0, ring0, cv, rings);
             if constexpr (!RING) {
 #pragma unroll
                 for (int i = 0; i < 2 * R + PF; ++i) ring0[i] = ring0[i + U];
@@ -432,7 +582,10 @@ struct Stencil2D {
         L.ylast = L.ye - 1 + T * R;
         const int y_lo = c0 - 2 * R;
         const int y_hi = min(c0 + nblk * U - 1 + PF, L.ylast);  // last row any tick loads
-        const bool interior = !L.edge && p.vec_ok && y_lo >= 0 && y_hi < p.height;
+        // (interior stores address a row through a buffer of width*sizeof(S) < 2^31
+        // bytes, with 32-bit offsets)
+        const bool interior = !L.edge && p.vec_ok && y_lo >= 0 && y_hi < p.height &&
+                              static_cast<long long>(p.width) * static_cast<long long>(sizeof(S)) < (1LL << 31);
         if (interior)
             stream<false>(L, w, c0, nblk);
         else

@@ -12,10 +12,12 @@ namespace ssam {
 
 // Per-storage-type geometry: columns per lane (one 16-byte access) for a single
 // sweep, and for fused time steps of radius 1 / radius 2 (fewer columns keep the
-// T-stage register rings in VGPRs), and the most steps fused per launch at
-// radius 2.
+// T-stage register rings in VGPRs and leave room for more waves per SIMD: FP32
+// FMAs reach their double rate only when two waves issue together, so FP32 T=8
+// radius 1 at 2 columns ran 10% faster than at 4), and the most steps fused per
+// launch at radius 2.
 template <typename S> struct Lanes2D;
-template <> struct Lanes2D<float> { static constexpr int V = 4, VT1 = 4, VT2 = 2, TMAX2 = 8; };
+template <> struct Lanes2D<float> { static constexpr int V = 4, VT1 = 2, VT2 = 2, TMAX2 = 8; };
 template <> struct Lanes2D<double> { static constexpr int V = 2, VT1 = 2, VT2 = 2, TMAX2 = 4; };
 template <> struct Lanes2D<bf16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
 template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
@@ -30,9 +32,12 @@ template <int R, int SHAPE, int V, int T> struct Unroll2D {
     static constexpr int K = 2 * R + 1;
     static constexpr int TAPS = SHAPE == kStar ? 4 * R + 1 : K * K;
     static constexpr int BODY = TAPS * V * T;
+    // fused steps: the input ring is two step rings long, so one ring period
+    // covers both
     static constexpr int PF_RING = T > 1 ? K : (R <= 2 ? 5 : 3);
     static constexpr int NR0 = K + PF_RING;
-    static constexpr int PERIOD = T > 1 ? (NR0 / cgcd(NR0, K)) * K : NR0;
+    static constexpr int NRS = K;
+    static constexpr int PERIOD = T > 1 ? (NR0 / cgcd(NR0, NRS)) * NRS : NR0;
     static constexpr bool RING = PERIOD * BODY <= 2048;
     static constexpr int U0 = 2048 / BODY;
     static constexpr int U = RING ? PERIOD : (U0 < 1 ? 1 : (U0 > 6 ? 6 : U0));
@@ -149,7 +154,16 @@ static int dispatch_radius(const Req2D& rq, hipStream_t st) {
 template <typename S, int SHAPE> int dispatch_shape(const Req2D& rq, hipStream_t st);
 #ifdef SSAM_2D_SHAPE_UNIT
 template <typename S, int SHAPE> int dispatch_shape(const Req2D& rq, hipStream_t st) {
-    return dispatch_radius<S, SHAPE, Lanes2D<S>::V, 1>(rq, st);
+    // 2-byte storage: 8 columns per lane up to radius 2; larger boxes on the VALU
+    // kernel (the matrix-core form takes them by default) use 4, whose ring and
+    // resident taps fit the register file
+    if constexpr (sizeof(S) == 2) {
+        if (rq.radius >= 3) return dispatch_radius<S, SHAPE, Lanes2D<S>::VT1, 1>(rq, st);
+        if (rq.radius == 1) return launch2d_instance<S, 1, SHAPE, Lanes2D<S>::V, 1>(rq, st);
+        return launch2d_instance<S, 2, SHAPE, Lanes2D<S>::V, 1>(rq, st);
+    } else {
+        return dispatch_radius<S, SHAPE, Lanes2D<S>::V, 1>(rq, st);
+    }
 }
 #endif
 
@@ -167,9 +181,18 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
             // lane) keep the ring in the loop-period form with room for more
             // waves; measured 13-36% faster than 8 columns at radius 5 and 6
             if constexpr (sizeof(S) == 2) {
-                if (rq.radius >= 4) return dispatch_radius<S, kStar, VT1, 1>(rq, st);
+                switch (rq.radius) {
+                    case 1: return launch2d_instance<S, 1, kStar, V, 1>(rq, st);
+                    case 2: return launch2d_instance<S, 2, kStar, V, 1>(rq, st);
+                    case 3: return launch2d_instance<S, 3, kStar, V, 1>(rq, st);
+                    case 4: return launch2d_instance<S, 4, kStar, VT1, 1>(rq, st);
+                    case 5: return launch2d_instance<S, 5, kStar, VT1, 1>(rq, st);
+                    case 6: return launch2d_instance<S, 6, kStar, VT1, 1>(rq, st);
+                    default: return SSAM_E_UNSUPPORTED;
+                }
+            } else {
+                return dispatch_radius<S, kStar, V, 1>(rq, st);
             }
-            return dispatch_radius<S, kStar, V, 1>(rq, st);
         }
         // FP64 storage accumulates in FP64: compensation changes nothing there
         if constexpr (sizeof(S) != 8) {
