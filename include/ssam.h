@@ -61,6 +61,12 @@ typedef enum {
                                     (reference 3D gold, tmp3d7pt-v2.cu:284-296) */
 } ssam_boundary;
 
+#define SSAM_FLAG_WORK_FP32 2u   /* BF16 / FP16 storage: `work` holds 2 * width *
+                                    height FP32 values (dense rows), where a run of
+                                    several fused launches keeps its running state
+                                    in FP32 between launches.  Without it such a
+                                    run is SSAM_E_INVALID (the state is never
+                                    rounded to the storage type). */
 #define SSAM_FLAG_COMPENSATED 1u /* Kahan-compensated accumulation: each filter
                                     row is summed on its own and added to the
                                     running sum with compensation.  BOX and DENSE
@@ -77,7 +83,8 @@ typedef struct {
                              (COMPENSATED: up to 19 x 19) */
     int32_t steps;        /* time steps applied (>= 1); > 1 only for ZERO boundary */
     int32_t boundary;     /* ssam_boundary: ZERO, CLAMP or HOLD */
-    uint32_t flags;       /* SSAM_FLAG_* (other bits: SSAM_E_INVALID) */
+    uint32_t flags;       /* SSAM_FLAG_COMPENSATED | SSAM_FLAG_WORK_FP32 (other bits:
+                             SSAM_E_INVALID) */
     const double* coeffs; /* HOST memory, fh*fw taps row-major: w[m*fw + n] weights
                              in(x + n - fw/2, y + m - fh/2) (correlation, no flip).
                              STAR reads only the centre row and column. */
@@ -106,7 +113,9 @@ const char* ssam_strerror(int status);
  * Applies `steps` sweeps of the 2D stencil / correlation described by `desc` to
  * src (width x height, row pitch `pitch`), result in dst.  Temporal steps are
  * fused in registers up to 8 per launch; a `steps` needing more than one launch
- * ping-pongs through `work` (same shape as dst; may be NULL otherwise).
+ * ping-pongs through `work` (may be NULL otherwise): F32 / F64 storage -- the
+ * same shape as dst; BF16 / F16 storage -- 2 * width * height FP32 values, with
+ * SSAM_FLAG_WORK_FP32 set (the running state stays FP32 between launches).
  * Replaces the kernel launches of stencil2D_5pt-v1.cu:194-195, stencil2D_*pt.cu,
  * stencil2d_temp_{5,9}pt.cu:146-147 and conv2D.cu:128-131. */
 int ssam_stencil2d(const ssam_stencil2d_desc* desc, const void* src, void* dst, int64_t width,

@@ -17,6 +17,7 @@ F32, F64, BF16, F16 = 0, 1, 2, 3
 STAR, BOX, DENSE, FACE_EDGE = 0, 1, 2, 3
 ZERO, CLAMP, HOLD, DIRICHLET0 = 0, 1, 2, 3
 FLAG_COMPENSATED = 1
+FLAG_WORK_FP32 = 2
 
 
 class SsamError(RuntimeError):

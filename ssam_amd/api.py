@@ -52,12 +52,25 @@ def _check_2d(src: torch.Tensor, dst: torch.Tensor) -> tuple[int, int, int]:
     return w, h, src.stride(0)
 
 
+def _work_flags(src: torch.Tensor, work: Optional[torch.Tensor], flags: int) -> int:
+    """BF16 / FP16 grids with a float32 `work` of 2 * height * width values: the
+    running state of multi-launch runs stays FP32 there (SSAM_FLAG_WORK_FP32)."""
+    if work is not None and src.element_size() == 2 and work.dtype == torch.float32:
+        h, w = src.shape
+        if work.numel() < 2 * h * w or not work.is_contiguous():
+            raise ValueError("a float32 work area for BF16/FP16 grids holds 2 * height * width values")
+        flags |= L.FLAG_WORK_FP32
+    return flags
+
+
 def stencil2d(src: torch.Tensor, dst: torch.Tensor, coeffs, shape: int = L.BOX, steps: int = 1,
               boundary: int = L.ZERO, work: Optional[torch.Tensor] = None, flags: int = 0,
               stream: Optional[torch.cuda.Stream] = None) -> None:
     """`steps` sweeps of the fh x fw correlation `coeffs` (row-major, taps weight
-    in(x + n - fw/2, y + m - fh/2)) over src -> dst (ssam_stencil2d)."""
+    in(x + n - fw/2, y + m - fh/2)) over src -> dst (ssam_stencil2d).  BF16 / FP16
+    runs of more than one fused launch need a float32 `work` of 2 * h * w values."""
     w, h, pitch = _check_2d(src, dst)
+    flags = _work_flags(src, work, flags)
     c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
     if c.ndim != 2:
         raise ValueError("coeffs must be a 2-D (fh, fw) array")
@@ -81,6 +94,7 @@ def tune2d(src: torch.Tensor, dst: torch.Tensor, coeffs, shape: int = L.BOX, ste
     (waves_per_simd, fused_steps_per_launch); later stencil2d calls with the same
     (storage, shape, radius, steps, width, height) use it.  dst/work are overwritten."""
     w, h, pitch = _check_2d(src, dst)
+    flags = _work_flags(src, work, flags)
     c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
     d = L.Stencil2DDesc()
     d.storage = dtype_code(src.dtype)

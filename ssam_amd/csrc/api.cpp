@@ -116,7 +116,7 @@ struct TapSrc {
 // tuned chunk) chained through work/dst, and the launches.
 int stencil2d_run(int storage, int kshape, int R, int steps, int boundary, const TapSrc& ts,
                   const void* src, void* dst, int64_t width, int64_t height, int64_t pitch,
-                  void* work, hipStream_t st) {
+                  void* work, bool work_fp32, hipStream_t st) {
     const int es = elem_size(storage);
     if (!es || !src || !dst) return SSAM_E_INVALID;
     if (width < 1 || height < 1 || pitch < width) return SSAM_E_INVALID;
@@ -139,9 +139,15 @@ int stencil2d_run(int storage, int kshape, int R, int steps, int boundary, const
         chunks.push_back(t);
         left -= t;
     }
+    // BF16 / FP16 chains keep the running state in FP32 between launches: `work`
+    // then holds two dense width x height FP32 grids (the states ping-pong there;
+    // the first launch reads src, the last writes dst)
+    const bool fp32_state = es == 2 && chunks.size() > 1;
+    if (fp32_state && !work_fp32) return SSAM_E_INVALID;  // work too small for FP32 state
+    const int64_t wbytes = fp32_state ? 2 * width * height * 4 : bytes;
     if (chunks.size() > 1) {
         if (!work) return SSAM_E_INVALID;
-        if (overlaps(work, bytes, dst, bytes) || overlaps(work, bytes, src, bytes))
+        if (overlaps(work, wbytes, dst, bytes) || overlaps(work, wbytes, src, bytes))
             return SSAM_E_INVALID;
     }
 
@@ -149,6 +155,8 @@ int stencil2d_run(int storage, int kshape, int R, int steps, int boundary, const
     rq.width = width;
     rq.height = height;
     rq.pitch = pitch;
+    rq.out_storage = storage;
+    rq.pitch_out = pitch;
     rq.shape = kshape;
     rq.radius = R;
     rq.boundary = boundary;
@@ -160,6 +168,21 @@ int stencil2d_run(int storage, int kshape, int R, int steps, int boundary, const
                                                                : kDefaultWavesPerSimd);
     rq.form = t_force_form > 0 ? t_force_form : env_int("SSAM_2D_FORM", tuned ? tv.form : kFormAuto);
     const size_t n = chunks.size();
+    if (fp32_state) {
+        float* state[2] = {static_cast<float*>(work), static_cast<float*>(work) + width * height};
+        for (size_t i = 0; i < n; ++i) {
+            const bool first = i == 0, last = i == n - 1;
+            rq.src = first ? src : state[(i - 1) % 2];
+            rq.dst = last ? dst : state[i % 2];
+            rq.pitch = first ? pitch : width;
+            rq.pitch_out = last ? pitch : width;
+            rq.out_storage = last ? storage : SSAM_F32;
+            rq.steps = chunks[i];
+            const int rc = launch2d(first ? storage : SSAM_F32, rq, st);
+            if (rc != SSAM_OK) return rc;
+        }
+        return SSAM_OK;
+    }
     const void* in = src;
     for (size_t i = 0; i < n; ++i) {
         // the last launch writes dst; earlier ones alternate so that holds
@@ -188,7 +211,7 @@ int kernel_form(const ssam_stencil2d_desc* d, int* kshape, int* radius, std::vec
         return SSAM_E_INVALID;
     if (d->boundary != SSAM_BOUNDARY_ZERO && d->steps != 1) return SSAM_E_INVALID;
     if (d->fw < 1 || d->fh < 1) return SSAM_E_INVALID;
-    if (d->flags & ~SSAM_FLAG_COMPENSATED) return SSAM_E_INVALID;
+    if (d->flags & ~(SSAM_FLAG_COMPENSATED | SSAM_FLAG_WORK_FP32)) return SSAM_E_INVALID;
     const bool comp = (d->flags & SSAM_FLAG_COMPENSATED) != 0;
     if (comp && (d->shape == SSAM_SHAPE_STAR || d->steps != 1)) return SSAM_E_UNSUPPORTED;
 
@@ -244,7 +267,7 @@ int stencil2d_impl(const ssam_stencil2d_desc* d, const void* src, void* dst, int
     if (rc != SSAM_OK) return rc;
     const TapSrc ts{taps.data(), nullptr};
     return stencil2d_run(d->storage, kshape, R, d->steps, d->boundary, ts, src, dst, width, height,
-                         pitch, work, st);
+                         pitch, work, (d->flags & SSAM_FLAG_WORK_FP32) != 0, st);
 }
 
 // ------------------------------------------------------------------ device tap copies
@@ -376,7 +399,7 @@ int conv_call(int dtype, const T* src, T* dst, int width, int width_stride, int 
         if (fw >= 2) {
             const TapSrc ts{nullptr, weight};
             return stencil2d_run(dtype, fw % 2 ? kBox : kBoxEven, fw / 2, 1, SSAM_BOUNDARY_ZERO, ts,
-                                 src, dst, width, height, width_stride, nullptr, st);
+                                 src, dst, width, height, width_stride, nullptr, false, st);
         }
     }
     // non-square and 1x1 filters: read the weights back and embed them on the host;
@@ -694,10 +717,12 @@ int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst
     const int fmax = max_fused(d->storage, d->shape, R);
     const int wps_c[] = {2, 4, 8, 12, 24};
     std::vector<int> tch_c;
+    const bool two_byte = d->storage == SSAM_BF16 || d->storage == SSAM_F16;
     for (int t = fmax; t >= 1; t >>= 1) {
         if (d->steps > 1 && t > d->steps) continue;
         const bool multi = (d->steps + t - 1) / t > 1;
         if (multi && !work) continue;
+        if (multi && two_byte && !(d->flags & SSAM_FLAG_WORK_FP32)) continue;
         tch_c.push_back(t);
         if (d->steps == 1) break;
     }

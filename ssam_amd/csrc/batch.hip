@@ -216,6 +216,7 @@ void tiles_for_2d(const ssam_batch_item& it, int idx, int seg_rows, Params2D* p,
     p->width = static_cast<int>(it.nx);
     p->height = static_cast<int>(it.ny);
     p->pitch = it.pitch;
+    p->pitch_out = it.pitch;
     p->nstrips = static_cast<int>((it.nx + I::G::OW - 1) / I::G::OW);
     p->seg_rows = seg_rows;
     p->boundary = kZero;

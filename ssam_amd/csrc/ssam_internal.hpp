@@ -19,6 +19,10 @@ struct Req2D {
     const void* src;
     void* dst;
     int64_t width, height, pitch;
+    // dst's storage (ssam_dtype; the request's storage when equal) and row pitch:
+    // FP32 for the running state between the fused launches of a BF16 / FP16 chain
+    int out_storage;
+    int64_t pitch_out;
     int shape;       // Shape2D (DENSE arrives embedded in a box)
     int radius;      // filter half-width after embedding
     int steps;       // fused steps of this launch

@@ -41,7 +41,8 @@ template <typename A, int NT> struct Taps { A w[NT]; };
 
 struct Params2D {
     int width, height;
-    long long pitch;  // elements between rows (src and dst)
+    long long pitch;      // elements between rows of src
+    long long pitch_out;  // elements between rows of dst (its own storage type)
     int nstrips;      // column strips
     int ntiles;       // nstrips * nsegs
     int seg_rows;     // output rows per tile
@@ -294,7 +295,10 @@ __device__ __forceinline__ void stencil_row(const A (&X)[2 * R + 1][V], const TW
 //    at its start, and after the block the rings shift down by U slots with
 //    register moves, which only touch rows whose loads had the whole block to
 //    land.  Used where a full ring period would not fit the instruction cache.
-template <typename S, int R, int SHAPE, int V, int T, int PF, bool RING, int U>
+// SO: the storage type of dst (default S).  A chain of fused launches over
+// BF16 / FP16 grids keeps its running state in FP32 between launches: its first
+// launch reads S and writes FP32, its last reads FP32 and writes S.
+template <typename S, int R, int SHAPE, int V, int T, int PF, bool RING, int U, typename SO = S>
 struct Stencil2D {
     using A = typename Storage<S>::acc;
     using G = Geom2D<R, V, T>;
@@ -311,11 +315,12 @@ struct Stencil2D {
 
     struct Lane {
         const S* __restrict__ src;
-        S* __restrict__ dst;
+        SO* __restrict__ dst;
         Params2D p;
         int x0, ys, ye;
         int ylast;      // last input row this tile needs (ye - 1 + T*R)
-        uint32_t xoff;  // byte offset of this lane's strip within a row (x0 >= 0)
+        uint32_t xoff;  // byte offset of this lane's strip within a src row (x0 >= 0)
+        uint32_t xoff_out;  // ... within a dst row
         bool lane_full, edge, store_lane;
         bool inb[V];
     };
@@ -373,30 +378,30 @@ struct Stencil2D {
         const bool ok = L.store_lane && y >= L.ys && y < L.ye;
         const int yy = y < 0 ? 0 : y;
         char* row = reinterpret_cast<char*>(L.dst) +
-                    static_cast<long long>(yy) * L.p.pitch * static_cast<long long>(sizeof(S));
-        S tmp[V];
+                    static_cast<long long>(yy) * L.p.pitch_out * static_cast<long long>(sizeof(SO));
+        SO tmp[V];
 #pragma unroll
-        for (int j = 0; j < V; ++j) tmp[j] = Storage<S>::store(v[j]);
-        constexpr int W = V * static_cast<int>(sizeof(S)) / 4;
+        for (int j = 0; j < V; ++j) tmp[j] = Storage<SO>::store(v[j]);
+        constexpr int W = V * static_cast<int>(sizeof(SO)) / 4;
         uint32_t words[W];
         __builtin_memcpy(words, tmp, sizeof(tmp));
-        buf_store<uint32_t, W>(buf_rsrc(row, static_cast<uint32_t>(L.p.width) * sizeof(S)),
-                               ok ? L.xoff : kBufOOB, words);
+        buf_store<uint32_t, W>(buf_rsrc(row, static_cast<uint32_t>(L.p.width) * sizeof(SO)),
+                               ok ? L.xoff_out : kBufOOB, words);
     }
 
     template <bool CHK>
     __device__ static __forceinline__ void store_row(const Lane& L, int y, const A (&v)[V]) {
         if (!L.store_lane) return;
         char* row = reinterpret_cast<char*>(L.dst) +
-                    static_cast<long long>(y) * L.p.pitch * static_cast<long long>(sizeof(S));
+                    static_cast<long long>(y) * L.p.pitch_out * static_cast<long long>(sizeof(SO));
         if (!CHK || (L.lane_full && L.p.vec_ok)) {
-            store_strip_vec<S, V>(reinterpret_cast<S*>(row + L.xoff), v);
+            store_strip_vec<SO, V>(reinterpret_cast<SO*>(row + L.xoff_out), v);
         } else {
 #pragma unroll
             for (int j = 0; j < V; ++j)
                 if (L.inb[j])
-                    gst<S>(row + L.xoff + static_cast<uint32_t>(j * sizeof(S)),
-                        Storage<S>::store(v[j]));
+                    gst<SO>(row + L.xoff_out + static_cast<uint32_t>(j * sizeof(SO)),
+                            Storage<SO>::store(v[j]));
         }
     }
 
@@ -555,7 +560,7 @@ struct Stencil2D {
 
     // One tile: column strip `strip`, output rows [seg*seg_rows, ...) of the grid
     // described by p.  strip and seg are wave-uniform.
-    __device__ static __forceinline__ void run_tile(const S* __restrict__ src, S* __restrict__ dst,
+    __device__ static __forceinline__ void run_tile(const S* __restrict__ src, SO* __restrict__ dst,
                                                     const Params2D& p, const A* __restrict__ w,
                                                     int strip, int seg) {
         const int lane = threadIdx.x & (kWave - 1);
@@ -566,6 +571,7 @@ struct Stencil2D {
         const int strip_x = strip * G::OW;
         L.x0 = strip_x + (lane - G::HL) * V;
         L.xoff = static_cast<uint32_t>(L.x0) * static_cast<uint32_t>(sizeof(S));
+        L.xoff_out = static_cast<uint32_t>(L.x0) * static_cast<uint32_t>(sizeof(SO));
         L.ys = seg * p.seg_rows;
         L.ye = min(p.height, L.ys + p.seg_rows);
         L.lane_full = L.x0 >= 0 && L.x0 + V <= p.width;
@@ -582,17 +588,17 @@ struct Stencil2D {
         L.ylast = L.ye - 1 + T * R;
         const int y_lo = c0 - 2 * R;
         const int y_hi = min(c0 + nblk * U - 1 + PF, L.ylast);  // last row any tick loads
-        // (interior stores address a row through a buffer of width*sizeof(S) < 2^31
+        // (interior stores address a row through a buffer of width*sizeof(SO) < 2^31
         // bytes, with 32-bit offsets)
         const bool interior = !L.edge && p.vec_ok && y_lo >= 0 && y_hi < p.height &&
-                              static_cast<long long>(p.width) * static_cast<long long>(sizeof(S)) < (1LL << 31);
+                              static_cast<long long>(p.width) * static_cast<long long>(sizeof(SO)) < (1LL << 31);
         if (interior)
             stream<false>(L, w, c0, nblk);
         else
             stream<true>(L, w, c0, nblk);
     }
 
-    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const Params2D& p,
+    __device__ static void run(const S* __restrict__ src, SO* __restrict__ dst, const Params2D& p,
                                const A* __restrict__ w) {
         // wave index is uniform across the wave: keep the tile bookkeeping in SGPRs
         const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -614,16 +620,17 @@ template <typename A, int R, int SHAPE> struct KernargTaps {
     static constexpr int count = fits ? n : 1;
 };
 
-template <typename S, int R, int SHAPE, int V, int T, int PF, bool RING, int U, int WPE>
+template <typename S, int R, int SHAPE, int V, int T, int PF, bool RING, int U, int WPE, typename SO = S>
 __global__ __launch_bounds__(kWave * kWavesPerBlock2D) __attribute__((amdgpu_waves_per_eu(WPE)))
 void ssam2d_kernel(
     Taps<typename Storage<S>::acc, KernargTaps<typename Storage<S>::acc, R, SHAPE>::count> tp,
-    const S* __restrict__ src, S* __restrict__ dst, Params2D p) {
+    const S* __restrict__ src, SO* __restrict__ dst, Params2D p) {
     (void)tp;
     using A = typename Storage<S>::acc;
+    static_assert(std::is_same<A, typename Storage<SO>::acc>::value, "one accumulation type");
     const A* __restrict__ w = p.taps ? static_cast<const A*>(p.taps)
                                      : (const A*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Stencil2D<S, R, SHAPE, V, T, PF, RING, U>::run(src, dst, p, w);
+    Stencil2D<S, R, SHAPE, V, T, PF, RING, U, SO>::run(src, dst, p, w);
 }
 
 }  // namespace ssam

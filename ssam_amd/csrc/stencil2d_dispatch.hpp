@@ -4,11 +4,20 @@
 #pragma once
 #include "stencil2d.hpp"
 #include "stencil2d_mfma.hpp"
+#include <type_traits>
 #include <vector>
 
 #include "ssam_internal.hpp"
 
 namespace ssam {
+
+// The ssam_dtype of a storage type.
+template <typename S> constexpr int storage_of() {
+    return std::is_same<S, float>::value    ? SSAM_F32
+           : std::is_same<S, double>::value ? SSAM_F64
+           : std::is_same<S, bf16_t>::value ? SSAM_BF16
+                                            : SSAM_F16;
+}
 
 // Per-storage-type geometry: columns per lane (one 16-byte access) for a single
 // sweep, and for fused time steps of radius 1 / radius 2 (fewer columns keep the
@@ -45,7 +54,7 @@ template <int R, int SHAPE, int V, int T> struct Unroll2D {
     static constexpr int WPE = 2;
 };
 
-template <typename S, int R, int SHAPE, int V, int T>
+template <typename S, int R, int SHAPE, int V, int T, typename SO = S>
 static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     using A = typename Storage<S>::acc;
     using G = Geom2D<R, V, T>;
@@ -73,11 +82,12 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     p.width = static_cast<int>(rq.width);
     p.height = static_cast<int>(rq.height);
     p.pitch = rq.pitch;
+    p.pitch_out = rq.pitch_out;
     p.nstrips = static_cast<int>((rq.width + G::OW - 1) / G::OW);
     p.boundary = rq.boundary;
     // one vector access per strip needs only element-aligned rows (any pitch)
     p.vec_ok = (reinterpret_cast<uintptr_t>(rq.src) % sizeof(S) == 0) &&
-               (reinterpret_cast<uintptr_t>(rq.dst) % sizeof(S) == 0);
+               (reinterpret_cast<uintptr_t>(rq.dst) % sizeof(SO) == 0);
     // rows per tile: enough tiles for ~target waves per SIMD, but at least a few
     // multiples of the halo so the 2*T*r re-read rows stay a small fraction
     int seg = rq.seg_rows;
@@ -98,9 +108,9 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     p.ntiles = static_cast<int>(ntiles);
     p.nblocks = (p.ntiles + kWavesPerBlock2D - 1) / kWavesPerBlock2D;
 
-    hipLaunchKernelGGL((ssam2d_kernel<S, R, SHAPE, V, T, PF, RING, U, WPE>), dim3(p.nblocks),
+    hipLaunchKernelGGL((ssam2d_kernel<S, R, SHAPE, V, T, PF, RING, U, WPE, SO>), dim3(p.nblocks),
                        dim3(kWave * kWavesPerBlock2D), 0, stream, tp, static_cast<const S*>(rq.src),
-                       static_cast<S*>(rq.dst), p);
+                       static_cast<SO*>(rq.dst), p);
     return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
 }
 
@@ -148,6 +158,51 @@ static int dispatch_radius(const Req2D& rq, hipStream_t st) {
     return SSAM_E_UNSUPPORTED;
 }
 
+// Fused steps (T in {2, 4, 8}) of a radius-1 or radius-2 star from S to SO; the
+// columns per lane are S's (the wider of the two element types decides nothing:
+// both sides of a mixed chain use the same tiling).
+template <typename S, typename SO>
+static int dispatch_fused(const Req2D& rq, hipStream_t st) {
+    using LN = Lanes2D<S>;
+    constexpr int VT1 = LN::VT1, VT2 = LN::VT2;
+    switch (rq.steps) {
+        case 1:
+            // the single-step tail of a mixed chain (steps not a power-of-two sum of 8s)
+            if constexpr (!std::is_same<S, SO>::value) {
+                return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT1, 1, SO>(rq, st)
+                                      : launch2d_instance<S, 2, kStar, VT2, 1, SO>(rq, st);
+            }
+            return SSAM_E_UNSUPPORTED;
+        case 2:
+            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT1, 2, SO>(rq, st)
+                                  : launch2d_instance<S, 2, kStar, VT2, 2, SO>(rq, st);
+        case 4:
+            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT1, 4, SO>(rq, st)
+                                  : launch2d_instance<S, 2, kStar, VT2, 4, SO>(rq, st);
+        case 8:
+            if (rq.radius == 1) return launch2d_instance<S, 1, kStar, VT1, 8, SO>(rq, st);
+            if constexpr (LN::TMAX2 >= 8) return launch2d_instance<S, 2, kStar, VT2, 8, SO>(rq, st);
+            return SSAM_E_UNSUPPORTED;
+        default:
+            return SSAM_E_UNSUPPORTED;
+    }
+}
+
+// The two halves of a BF16 / FP16 chain of fused launches: 2-byte storage in,
+// FP32 running state out (S = bf16/f16), and FP32 in, 2-byte storage out
+// (S = float).  Radius-1 and radius-2 stars only.
+template <typename S>
+static int dispatch_mixed(const Req2D& rq, hipStream_t st) {
+    if (rq.shape != kStar || rq.radius > 2) return SSAM_E_UNSUPPORTED;
+    if constexpr (sizeof(S) == 2) {
+        if (rq.out_storage == SSAM_F32) return dispatch_fused<S, float>(rq, st);
+    } else if constexpr (std::is_same<S, float>::value) {
+        if (rq.out_storage == SSAM_BF16) return dispatch_fused<float, bf16_t>(rq, st);
+        if (rq.out_storage == SSAM_F16) return dispatch_fused<float, f16_t>(rq, st);
+    }
+    return SSAM_E_UNSUPPORTED;
+}
+
 // One sweep of a box-family shape (kBox, kBoxKahan, kBoxEven) on the VALU kernel.
 // Each (storage, shape) has its own compile unit (stencil2d_<shape>_<type>.hip,
 // SSAM_INSTANTIATE_2D_SHAPE), so the large-radius instances build in parallel.
@@ -169,7 +224,8 @@ template <typename S, int SHAPE> int dispatch_shape(const Req2D& rq, hipStream_t
 
 template <typename S>
 int dispatch2d(const Req2D& rq, hipStream_t st) {
-    constexpr int V = Lanes2D<S>::V, VT1 = Lanes2D<S>::VT1, VT2 = Lanes2D<S>::VT2;
+    constexpr int V = Lanes2D<S>::V, VT1 = Lanes2D<S>::VT1;
+    if (rq.out_storage != storage_of<S>()) return dispatch_mixed<S>(rq, st);
     if (rq.steps == 1) {
         // matrix cores: by default from radius 2 (where they beat the VALU kernels
         // on MI355X); the autotuner may pick them for radius 1 too
@@ -208,20 +264,7 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
     // fused time steps: star radius 1 and 2 (the reference's temporal 5-point and
     // 9-point star kernels), T in {2, 4, 8}
     if (rq.shape != kStar || rq.radius > 2) return SSAM_E_UNSUPPORTED;
-    switch (rq.steps) {
-        case 2:
-            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT1, 2>(rq, st)
-                                  : launch2d_instance<S, 2, kStar, VT2, 2>(rq, st);
-        case 4:
-            return rq.radius == 1 ? launch2d_instance<S, 1, kStar, VT1, 4>(rq, st)
-                                  : launch2d_instance<S, 2, kStar, VT2, 4>(rq, st);
-        case 8:
-            if (rq.radius == 1) return launch2d_instance<S, 1, kStar, VT1, 8>(rq, st);
-            if constexpr (Lanes2D<S>::TMAX2 >= 8) return launch2d_instance<S, 2, kStar, VT2, 8>(rq, st);
-            return SSAM_E_UNSUPPORTED;
-        default:
-            return SSAM_E_UNSUPPORTED;
-    }
+    return dispatch_fused<S, S>(rq, st);
 }
 
 }  // namespace ssam

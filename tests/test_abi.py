@@ -81,8 +81,13 @@ def test_stencil2d_rejects_malformed_requests():
 
 def test_stencil2d_flags():
     dd, _ = _desc(shape=L.BOX)
-    dd.flags = 2                                            # unknown flag bit
+    dd.flags = 4                                            # unknown flag bit
     assert _s2d(dd) == L.E_INVALID
+    # BF16 / FP16 runs of several fused launches keep their state in FP32: without
+    # the flag that declares an FP32 work area they are rejected before any launch
+    for sto in (L.BF16, L.F16):
+        dd, _ = _desc(storage=sto, steps=16)
+        assert _s2d(dd, work=FAKE_W) == L.E_INVALID
     dd, _ = _desc(shape=L.STAR)                             # compensated star
     dd.flags = L.FLAG_COMPENSATED
     assert _s2d(dd) == L.E_UNSUPPORTED

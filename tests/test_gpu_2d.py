@@ -246,6 +246,36 @@ def test_storage_precision_vs_fp64(storage, tol, shape, r, steps):
     assert_close(got, want, scale_of(x, f, steps), tol, f"{storage} r={r} T={steps}")
 
 
+@pytest.mark.parametrize("storage,tol", [(torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
+@pytest.mark.parametrize("r,steps", [(1, 16), (2, 16), (1, 13), (2, 11)])
+def test_storage_precision_multilaunch_fp32_state(storage, tol, r, steps):
+    # more steps than one launch fuses: the running state between launches stays
+    # FP32 in a float32 work area of 2*h*w values, so the result carries the input
+    # and output rounding only, at every step count
+    f = wl.normalised_taps("star", r, 78)
+    h, w = 150, 700
+    x = oracle.uniform(6, h * w, -1, 1, np.float64).reshape(h, w)
+    src = dev(x, dtype=storage)
+    dst = torch.full_like(src, float("nan"))
+    work = torch.empty(2 * h * w, dtype=torch.float32, device="cuda")
+    want = oracle.conv2d_iter(x, f, steps)
+    sc = scale_of(x, f, steps)
+    api.stencil2d(src, dst, f, shape=L.STAR, steps=steps, work=work)
+    torch.cuda.synchronize()
+    assert_close(dst.double().cpu().numpy(), want, sc, tol, f"{storage} r={r} T={steps}")
+    # the tuner's smaller fused chunks chain through the same FP32 state
+    api.tune2d(src, dst, f, shape=L.STAR, steps=steps, work=work)
+    dst.fill_(float("nan"))
+    api.stencil2d(src, dst, f, shape=L.STAR, steps=steps, work=work)
+    torch.cuda.synchronize()
+    assert_close(dst.double().cpu().numpy(), want, sc, tol, f"{storage} r={r} T={steps} tuned")
+    api.tune_clear()
+    # a work area of the storage type cannot hold the FP32 state: rejected, never rounded
+    with pytest.raises(L.SsamError) as e:
+        api.stencil2d(src, dst, f, shape=L.STAR, steps=steps, work=torch.empty_like(src))
+    assert e.value.status == L.E_INVALID
+
+
 # ------------------------------------------------------------------ full size
 def _band_check(x, got, f, rows, dt, steps=1):
     r = f.shape[0] // 2 * steps
