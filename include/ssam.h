@@ -206,6 +206,18 @@ int ssam_poisson_f64(const double* src, double* dst, int n, int iters, double* w
 int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz,
                   int steps, void* work, double* residual_l2, void* stream);
 
+/* ssam_jacobi19 for callers that cannot allocate (code under hipGraph stream
+ * capture, tight serving loops): no counterpart in the reference.  Same sweeps
+ * and bit-identical results, but it allocates nothing -- the residual's per-tile
+ * partials go to a fixed 1 MiB device array inside the library (one per GPU,
+ * held until the process ends), grids needing more than 131072 partials are
+ * SSAM_E_UNSUPPORTED -- and `residual_l2` is a DEVICE double (may be NULL)
+ * receiving the FP64 L2 norm, so the call is asynchronous.  Calls that overlap
+ * on one GPU share that array and race; the first call on a device looks up the
+ * array's address, so make it outside graph capture. */
+int ssam_jacobi19_noalloc(int storage, const void* src, void* dst, int64_t nx, int64_t ny,
+                          int64_t nz, int steps, void* work, double* residual_l2, void* stream);
+
 /* ---------------------------------------------------------------- batches
  * Many independent grids of uneven sizes and mixed shapes.  2D items of one
  * storage type whose shape is a 5-point star or a 3x3 / 5x5 / 11x11 box run as

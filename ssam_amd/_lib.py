@@ -103,6 +103,7 @@ _PROTOS = {
     "ssam_poisson_f32": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "ssam_poisson_f64": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "ssam_jacobi19": (_i, [_i, _vp, _vp, _i64, _i64, _i64, _i, _vp, ctypes.POINTER(_d), _vp]),
+    "ssam_jacobi19_noalloc": (_i, [_i, _vp, _vp, _i64, _i64, _i64, _i, _vp, _vp, _vp]),
     "ssam_tune_stencil2d": (_i, [ctypes.POINTER(Stencil2DDesc), _vp, _vp, _i64, _i64, _i64, _vp, _vp,
                                  ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "ssam_tune_clear": (None, []),

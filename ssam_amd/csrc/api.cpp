@@ -540,9 +540,20 @@ int get_scratch(int nwords, double** out) {
     return SSAM_OK;
 }
 
+// Where a residual request keeps its per-tile partials: the library's grown
+// device scratch (ssam_jacobi19; the norm comes back to the host) or a fixed
+// area the caller resolved beforehand (ssam_jacobi19_noalloc; the norm stays on
+// the device).
+struct ResidualArea {
+    double* partials;  // nullptr: the grown scratch
+    int cap;           // partials the area holds
+    double* out_dev;   // device norm (fixed area only)
+};
+
 int stencil3d_run(int storage, int shape, int radius, const double* taps, int steps,
                   const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz, void* work,
-                  double* residual_l2, double res_scale, hipStream_t st) {
+                  double* residual_l2, double res_scale, hipStream_t st,
+                  const ResidualArea* fixed = nullptr) {
     if (!src || !dst || !taps) return SSAM_E_INVALID;
     if (storage != SSAM_F32 && storage != SSAM_F64) {
         if (storage == SSAM_BF16 || storage == SSAM_F16) return SSAM_E_UNSUPPORTED;
@@ -565,6 +576,7 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
         if (overlaps(work, bytes, dst, bytes) || overlaps(work, bytes, src, bytes))
             return SSAM_E_INVALID;
     }
+    const bool want_res = fixed ? fixed->out_dev != nullptr : residual_l2 != nullptr;
     Req3D rq;
     rq.nx = nx;
     rq.ny = ny;
@@ -576,51 +588,60 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
     rq.partials = nullptr;
     rq.partials_cap = 0;
     rq.res_out = nullptr;
+    rq.res_sqrt = 0;
     rq.res_scale = res_scale;
     double* scratch = nullptr;
     int nparts = 0;
     std::unique_lock<std::mutex> scratch_lock;
-    if (residual_l2) {
-        scratch_lock = std::unique_lock<std::mutex>(g_scratch.mu);
+    if (want_res) {
         nparts = partials_needed(storage, rq);
-        const int rc = get_scratch(nparts + 1, &scratch);
-        if (rc != SSAM_OK) return rc;
+        if (fixed) {
+            if (!fixed->partials) return SSAM_E_INVALID;
+            if (nparts > fixed->cap) return SSAM_E_UNSUPPORTED;
+        } else {
+            scratch_lock = std::unique_lock<std::mutex>(g_scratch.mu);
+            const int rc = get_scratch(nparts + 1, &scratch);
+            if (rc != SSAM_OK) return rc;
+        }
     }
     // launches: two sweeps each where the temporal form applies (radius 1, grid
     // under 2 GiB, SSAM_3D_T != 1), the rest one sweep; a residual request runs
     // the last sweep on its own (the single-sweep kernel forms the residual).
     // Launches ping-pong so the last one writes dst.
-    std::vector<int> sweeps;
-    {
-        const bool t2 = steps >= 2 && two_sweeps_ok(storage, rq) && env_int("SSAM_3D_T", 2) != 1;
-        int left = steps;
-        const int tail = residual_l2 ? 1 : 0;
-        while (left > 0) {
-            const int n = (t2 && left - tail >= 2) ? 2 : 1;
-            sweeps.push_back(n);
-            left -= n;
-        }
-    }
-    const int nl = static_cast<int>(sweeps.size());
+    const bool t2 = steps >= 2 && two_sweeps_ok(storage, rq) && env_int("SSAM_3D_T", 2) != 1;
+    const int tail = want_res ? 1 : 0;
+    auto sweeps_of = [&](int left) { return (t2 && left - tail >= 2) ? 2 : 1; };
+    int nl = 0;
+    for (int left = steps; left > 0; left -= sweeps_of(left)) ++nl;
     const void* in = src;
+    int left = steps;
     for (int i = 0; i < nl; ++i) {
+        const int n = sweeps_of(left);
+        left -= n;
         void* out = ((nl - 1 - i) % 2 == 0) ? dst : work;
         rq.src = in;
         rq.dst = out;
-        if (residual_l2 && i == nl - 1) {
-            rq.partials = scratch + 1;
-            rq.partials_cap = nparts;
-            rq.res_out = scratch;
+        if (want_res && i == nl - 1) {
+            if (fixed) {
+                rq.partials = fixed->partials;
+                rq.partials_cap = fixed->cap;
+                rq.res_out = fixed->out_dev;
+                rq.res_sqrt = 1;
+            } else {
+                rq.partials = scratch + 1;
+                rq.partials_cap = nparts;
+                rq.res_out = scratch;
+            }
         }
-        const int rc = sweeps[i] == 2 ? dispatch3d_two_sweeps(storage, rq, st) : dispatch3d(storage, rq, st);
+        const int rc = n == 2 ? dispatch3d_two_sweeps(storage, rq, st) : dispatch3d(storage, rq, st);
         if (rc != SSAM_OK) {
             // launches already queued may use the scratch: let them finish first
-            if (residual_l2) (void)hipStreamSynchronize(st);
+            if (want_res && !fixed) (void)hipStreamSynchronize(st);
             return rc;
         }
         in = out;
     }
-    if (residual_l2) {
+    if (want_res && !fixed) {
         double h = 0.0;
         if (hipMemcpyAsync(&h, scratch, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
@@ -680,6 +701,60 @@ int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t n
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ no-allocation Jacobi
+// The residual partials of ssam_jacobi19_noalloc live in this fixed device array
+// (each GPU the library runs on holds its own copy until the process ends).  In
+// host code `scratch` only names the symbol: its device address comes from
+// hipGetSymbolAddress, once per device, on that device's first call, and only
+// that address is ever passed to a kernel.  Every call on one GPU shares the
+// array: calls that overlap on a GPU race.  Nothing in it needs clearing -- each
+// call's sweep writes every partial its sum reads.
+namespace {
+
+constexpr int kNoAllocPartials = 1 << 17;  // 1 MiB of FP64 partials
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[64] = {};
+
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return SSAM_E_HIP;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SSAM_E_HIP;
+        g_noalloc_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return SSAM_OK;
+}
+
+}  // namespace
+
+extern "C" int ssam_jacobi19_noalloc(int storage, const void* src, void* dst, int64_t nx,
+                                     int64_t ny, int64_t nz, int steps, void* work,
+                                     double* residual_l2, void* stream) {
+    ResidualArea area{nullptr, kNoAllocPartials, residual_l2};
+    const bool valid = src && dst && (storage == SSAM_F32 || storage == SSAM_F64) && nx >= 1 &&
+                       ny >= 1 && nz >= 1 && nx <= (1 << 30) && ny <= (1 << 30) && nz <= (1 << 30);
+    if (residual_l2 && valid) {
+        // grids whose partials outgrow the array are rejected before any device work
+        Req3D rq{};
+        rq.nx = nx;
+        rq.ny = ny;
+        rq.nz = nz;
+        rq.shape = 1;
+        rq.radius = 1;
+        rq.zseg = env_int("SSAM_ZSEG", 0);
+        if (partials_needed(storage, rq) > kNoAllocPartials) return SSAM_E_UNSUPPORTED;
+        const int rc = noalloc_scratch(&area.partials);
+        if (rc != SSAM_OK) return rc;
+    }
+    return stencil3d_run(storage, SSAM_SHAPE_FACE_EDGE, 1, kJacobi19, steps, src, dst, nx, ny, nz,
+                         work, nullptr, 4.0, static_cast<hipStream_t>(stream), &area);
+}
 
 // ------------------------------------------------------------------ autotuner
 namespace {

@@ -59,7 +59,8 @@ struct Req3D {
     int zseg;            // output planes per tile, 0 = heuristic
     double* partials;    // device, per-tile residual partials (nullptr: none)
     int partials_cap;
-    double* res_out;     // device, the summed residual^2
+    double* res_out;     // device, the summed residual^2 (or its root, res_sqrt)
+    int res_sqrt;
     double res_scale;
 };
 

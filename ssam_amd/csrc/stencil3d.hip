@@ -2,7 +2,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
-#include <vector>
 
 #include "ssam_internal.hpp"
 #include "stencil3d.hpp"
@@ -10,9 +9,10 @@
 
 namespace ssam {
 
-// fixed-order sum of the per-tile partials (one workgroup)
+// fixed-order sum of the per-tile partials (one workgroup); out[0] = the sum, or
+// its square root (the L2 norm) when take_sqrt is set
 __global__ __launch_bounds__(256) void sum_partials_kernel(const double* __restrict__ part, int n,
-                                                           double* __restrict__ out) {
+                                                           double* __restrict__ out, int take_sqrt) {
     __shared__ double sh[256];
     double s = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) s += part[i];
@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double* __restr
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = sh[0];
+    if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(sh[0]) : sh[0];
 }
 
 template <typename S> struct Lanes3D;
@@ -73,7 +73,7 @@ static int launch3d(const Req3D& rq, hipStream_t st) {
     if (hipGetLastError() != hipSuccess) return SSAM_E_HIP;
     if (rq.partials) {
         hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, rq.partials,
-                           p.ntiles, rq.res_out);
+                           p.ntiles, rq.res_out, rq.res_sqrt);
         if (hipGetLastError() != hipSuccess) return SSAM_E_HIP;
     }
     return SSAM_OK;

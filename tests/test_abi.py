@@ -200,3 +200,16 @@ def test_microbench_validates():
     assert L.lib().ssam_microbench(-1, ctypes.byref(v), None) == L.E_INVALID
     assert L.lib().ssam_microbench(18, ctypes.byref(v), None) == L.E_INVALID
     assert L.lib().ssam_microbench(0, None, None) == L.E_INVALID
+
+
+def test_jacobi19_noalloc_validates_without_a_device():
+    """The allocation-free Jacobi entry rejects bad requests before any device
+    work: null pointers, 2-byte storage, and grids whose residual partials would
+    outgrow its fixed array (131072 tiles)."""
+    lib = L.lib()
+    res = ctypes.c_void_p(0x20000000)
+    assert lib.ssam_jacobi19_noalloc(L.F32, None, FAKE_B, 64, 64, 64, 1, None, res, None) == L.E_INVALID
+    assert lib.ssam_jacobi19_noalloc(L.BF16, FAKE_A, FAKE_B, 64, 64, 64, 1, None, res, None) == L.E_UNSUPPORTED
+    big = ctypes.c_void_p(0x900000000)
+    assert lib.ssam_jacobi19_noalloc(L.F32, FAKE_A, big, 1 << 16, 1 << 14, 8, 1, None, res, None) == \
+        L.E_UNSUPPORTED
