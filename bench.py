@@ -9,10 +9,10 @@ U[0,1) normalised to sum 1, inputs resident in HBM.  value = total point-updates
 time of the step, in Gpoints/s.
 
 Also measured in the same run (outside the timed steps): per-kernel times with
-HIP events on the launch stream, the stream-copy bandwidth, the same suite in FP64
-and in BF16 storage (with max / RMS error of BF16 against the GPU FP64 path on
-identical inputs), and the CPU baseline (the oracle on a bounded slab, all host
-threads).
+HIP events on the launch stream and each kernel's binding roof (HBM or FP32
+compute), the stream-copy bandwidth, the same suite in FP64 and in BF16 / FP16
+storage, errors of every form against the FP64 oracle on row bands, configs 0 and
+2-4, and the CPU baseline (the oracle on a bounded slab, all host threads).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2]
 """
@@ -29,7 +29,12 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+# MI355X published dense peaks (TFLOP/s): FP32 (vector and matrix), FP64, BF16/FP16 matrix
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 N = 8192
+BANDS = (0, 2040, 4093, 8176)  # first rows of the 16-row bands checked against the oracle
 
 # (name, shape, radius, fused steps)
 SUITE = [
@@ -191,6 +196,53 @@ def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
     print("\n".join(md))
 
 
+def roof(flops: float, nbytes: float, ms: float, peak_tflops: float) -> dict:
+    """The binding roof of one kernel call: HBM (algorithmic bytes at 8 TB/s) or
+    compute (algorithmic flops at the published peak), whichever takes longer;
+    frac = that time / the measured time."""
+    t_hbm = nbytes / (HBM_PEAK_GBPS * 1e9)
+    t_cmp = flops / (peak_tflops * 1e12)
+    bound = "mfma" if t_cmp > t_hbm else "hbm"
+    t_roof = max(t_hbm, t_cmp)
+    achieved = (flops / (ms * 1e-3) / 1e12) if bound == "mfma" else (nbytes / (ms * 1e-3) / 1e9)
+    return dict(bound=bound, flop_per_byte=round(flops / nbytes, 2),
+                ridge=round(peak_tflops * 1e12 / (HBM_PEAK_GBPS * 1e9), 2),
+                achieved=round(achieved, 1), unit="TFLOP/s" if bound == "mfma" else "GB/s",
+                peak=peak_tflops if bound == "mfma" else HBM_PEAK_GBPS,
+                frac=round(t_roof / (ms * 1e-3), 4))
+
+
+def band_errors(got, seed: int, a: float, b: float, f, steps: int = 1, bands=BANDS) -> dict:
+    """Error of a device result against the FP64 oracle on 16-row bands: the
+    input rows each band needs are regenerated on the host (the same splitmix64
+    stream as the device fill) and run through the oracle in FP64; max and RMS
+    absolute error, and max |error| / max |gold| over the bands."""
+    import numpy as np
+
+    from oracle import oracle
+
+    n_rows, n_cols = got.shape
+    rt = (f.shape[0] // 2) * steps
+    mx, sq, cnt, gmax = 0.0, 0.0, 0, 0.0
+    for y0 in bands:
+        y0 = min(y0, n_rows - 16)
+        lo, hi = max(0, y0 - rt), min(n_rows, y0 + 16 + rt)
+        x = oracle.uniform(seed, (hi - lo) * n_cols, a, b, np.float64, offset=lo * n_cols).reshape(hi - lo, n_cols)
+        want = oracle.conv2d_iter(x, f, steps) if steps > 1 else oracle.conv2d(x, np.asarray(f, np.float64))
+        a0 = y0 - lo
+        if lo > 0 and a0 < rt:
+            continue  # the slab's artificial top edge would reach the band
+        w = want[a0:a0 + 16]
+        g = got[y0:y0 + 16].double().cpu().numpy()
+        d = np.abs(g - w)
+        mx = max(mx, float(d.max()))
+        sq += float((d * d).sum())
+        cnt += d.size
+        gmax = max(gmax, float(np.abs(w).max()))
+    return dict(max_abs_err=mx, rms_err=(sq / max(cnt, 1)) ** 0.5, max_rel_err_normwise=mx / max(gmax, 1e-300),
+                checked_points=cnt, against="FP64 oracle on row bands " + ",".join(map(str, bands)))
+
+
 def main():
     a = parse()
     if a.ingest_pmc:
@@ -282,18 +334,24 @@ def main():
         ms = float(np.mean([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(a.steps)]))
         upd = pts * t
         tw, tc = tuning.get(name, (0, 0))
-        byts = 2 * 4 * pts  # each point read once and written once per launch
+        # launches actually issued: the steps split into fused chunks of the tuned size
+        launches = (t + tc - 1) // tc if t > 1 and tc else 1
+        byts = 2 * 4 * pts * launches  # each point read once and written once per launch
         taps_n = (4 * r + 1) if shape == "star" else (2 * r + 1) ** 2
+        flops = upd * (2 * taps_n - 1)
         per.append(dict(name=name, ms=round(ms, 4), gpts=round(upd / ms / 1e6, 2),
-                        gbps=round(byts / ms / 1e6, 1),
+                        launches=launches, gbps=round(byts / ms / 1e6, 1),
                         hbm_frac=round(byts / ms / 1e6 / HBM_PEAK_GBPS, 3),
-                        gflops=round(upd * (2 * taps_n - 1) / ms / 1e6, 1),
+                        tflops=round(flops / ms / 1e9, 2),
+                        roofline=roof(flops, byts, ms, FP32_PEAK_TFLOPS),
                         tuned_waves_per_simd=tw, tuned_steps_per_launch=tc))
     updates_per_step = sum(pts * t for (_, _, _, t) in suite)
     value = updates_per_step / (ms_per_step * 1e-3) / 1e9
 
+    # the dominant kernel: the largest share of the step's time
     dom = max(per, key=lambda p: p["ms"])
-    dom_bytes = 2 * 4 * pts
+    di = [p["name"] for p in per].index(dom["name"])
+    d_shape, d_r, d_t = suite[di][1], suite[di][2], suite[di][3]
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{dom['name']}_8192_f32.json"
     if tfile.exists():
@@ -301,10 +359,20 @@ def main():
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
-    roofline = dict(bound="hbm", kernel=f"{dom['name']} f32 8192^2",
-                    achieved=round(dom_bytes / (dom["ms"] * 1e-3) / 1e9, 1), peak=HBM_PEAK_GBPS,
-                    unit="GB/s", frac=round(dom_bytes / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                    traffic=traffic, algorithmic_bytes=dom_bytes)
+    dr = dom["roofline"]
+    roofline = dict(bound=dr["bound"], kernel=f"{dom['name']} f32 8192^2", achieved=dr["achieved"],
+                    peak=dr["peak"], unit=dr["unit"], frac=dr["frac"], traffic=traffic,
+                    algorithmic_bytes=2 * 4 * pts, algorithmic_flops=int(
+                        pts * d_t * (2 * ((4 * d_r + 1) if d_shape == "star" else (2 * d_r + 1) ** 2) - 1)),
+                    launches=dom["launches"], ms=dom["ms"],
+                    note="frac = time at the binding roof (HBM 8 TB/s or FP32 157.3 TFLOP/s, whichever "
+                         "is longer for the algorithmic bytes / flops) over the measured time")
+    # FP32 suite errors against the FP64 oracle (bands of the 8192^2 output)
+    for i, (name, shape, r, t) in enumerate(suite):
+        api.stencil2d(src, dst, taps(shape, r), shape=L.STAR if shape == "star" else L.BOX,
+                      steps=t, work=work, stream=stream)
+        torch.cuda.synchronize()
+        per[i]["error_vs_fp64_oracle"] = band_errors(dst, 0, -1.0, 1.0, taps(shape, r), t)
 
     # ---------------------------------------------------------------- stream bandwidth
     cp_a = torch.empty(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
@@ -370,7 +438,7 @@ def main():
 def side_runs(suite, taps, stream, per_f32, tune=True):
     """FP64 and BF16/FP16-storage versions of the suite (same input values), each
     kernel autotuned (tiling, fused steps per launch, VALU or matrix-core form) and
-    timed with events; BF16/FP16 errors against the GPU FP64 results."""
+    timed with events; errors against the FP64 oracle on row bands."""
     import numpy as np
     import torch
 
@@ -381,7 +449,6 @@ def side_runs(suite, taps, stream, per_f32, tune=True):
     n = N
     x64 = torch.empty((n, n), dtype=torch.float64, device="cuda")
     api.fill_uniform(x64, 0, -1.0, 1.0)
-    res = {}
     for label, dt in (("fp64", torch.float64), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
         src = x64.to(dt)
         dst = torch.empty_like(src)
@@ -402,15 +469,7 @@ def side_runs(suite, taps, stream, per_f32, tune=True):
             ms = e0.elapsed_time(e1) / 3
             entry = dict(name=name, ms=round(ms, 4), gpts=round(n * n * t / ms / 1e6, 2),
                          speedup_vs_f32=round(per_f32[i]["ms"] / ms, 3))
-            if label == "fp64":
-                res[name] = dst.clone()
-            else:
-                ref = res[name]
-                d = (dst.to(torch.float64) - ref).abs()
-                denom = ref.abs().max()
-                entry["max_abs_err"] = float(d.max())
-                entry["rms_err"] = float(torch.sqrt((d * d).mean()))
-                entry["max_rel_err_normwise"] = float(d.max() / denom)
+            entry.update(band_errors(dst, 0, -1.0, 1.0, f, t))
             kern.append(entry)
         upd = sum(n * n * t for (_, _, _, t) in suite)
         tot = sum(k["ms"] for k in kern)
@@ -434,12 +493,15 @@ def _time(fn, stream, reps=3):
 
 def side_config0(stream, copy_gbps):
     """Config 0: one sweep of the 5-point star (centre 0.5, N/S/E/W 0.125) on 2048^2
-    FP32, values U[0,1) from seed 0, the 1-cell border held fixed (HOLD).  Parity
-    with the gold at this exact size is a GPU test; the 32 MiB working set stays in
-    the last-level cache across repetitions, so the rate is not an HBM figure."""
+    FP32, values U[0,1) from seed 0, the 1-cell border held fixed (HOLD).  The
+    32 MiB working set fits the 256 MB last-level cache, so it is timed two ways:
+    cold -- a 1 GiB copy between repetitions evicts it, each sweep timed alone --
+    and hot -- 20 back-to-back sweeps replayed from a graph.  The error is against
+    the FP64 oracle on every point."""
     import numpy as np
     import torch
 
+    from oracle import oracle
     from ssam_amd import _lib as L
     from ssam_amd import api
 
@@ -449,10 +511,53 @@ def side_config0(stream, copy_gbps):
     y = torch.empty_like(x)
     f = np.array([[0.0, 0.125, 0.0], [0.125, 0.5, 0.125], [0.0, 0.125, 0.0]])
     api.tune2d(x, y, f, shape=L.STAR, boundary=L.HOLD, stream=stream)
-    ms = _time(lambda: api.stencil2d(x, y, f, shape=L.STAR, boundary=L.HOLD, stream=stream), stream, reps=20)
-    gbps = 8 * n * n / ms / 1e6
-    return dict(ms=round(ms, 4), gpts=round(n * n / ms / 1e6, 2), gbps=round(gbps, 1),
-                stream_frac=round(gbps / copy_gbps, 3))
+    run = lambda: api.stencil2d(x, y, f, shape=L.STAR, boundary=L.HOLD, stream=stream)  # noqa: E731
+    # cold: evict the LLC before every timed sweep
+    fa = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    fb = torch.empty_like(fa)
+    cold = []
+    for _ in range(12):
+        api.stream_copy(fa, fb, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        cold.append(e0.elapsed_time(e1))
+    del fa, fb
+    ms_cold = float(np.median(cold[2:]))
+    # hot: back-to-back sweeps from one graph
+    run()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(20):
+            api.stencil2d(x, y, f, shape=L.STAR, boundary=L.HOLD)
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(5):
+        g.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_hot = e0.elapsed_time(e1) / 100
+    run()
+    torch.cuda.synchronize()
+    xs = oracle.uniform(0, n * n, 0.0, 1.0, np.float64).reshape(n, n)
+    want = oracle.conv2d(xs, f)
+    want[0, :], want[-1, :], want[:, 0], want[:, -1] = xs[0, :], xs[-1, :], xs[:, 0], xs[:, -1]
+    d = np.abs(y.double().cpu().numpy() - want)
+    out = {}
+    for label, ms in (("cold", ms_cold), ("hot", ms_hot)):
+        gbps = 8 * n * n / ms / 1e6
+        out[label] = dict(ms=round(ms, 4), gpts=round(n * n / ms / 1e6, 2), gbps=round(gbps, 1),
+                          hbm_frac=round(gbps / HBM_PEAK_GBPS, 3), stream_frac=round(gbps / copy_gbps, 3))
+    out["cold"]["note"] = "LLC evicted by a 1 GiB copy before each sweep; median of 10"
+    out["hot"]["note"] = "20 back-to-back sweeps per graph replay (the 32 MiB set stays in the LLC)"
+    out.update(max_abs_err=float(d.max()), rms_err=float(np.sqrt((d * d).mean())),
+               max_rel_err_normwise=float(d.max() / np.abs(want).max()), against="FP64 oracle, every point")
+    return out
 
 
 def side_3d(stream, copy_gbps):
@@ -477,8 +582,12 @@ def side_3d(stream, copy_gbps):
                                      ("poisson19", L.FACE_EDGE, 1, wl.REF3D_POISSON)):
             ms = _time(lambda: api.stencil3d(src, dst, coef, shape=shape, radius=r, stream=stream), stream)
             gbps = 2 * es * n ** 3 / ms / 1e6
-            kern.append(dict(name=name, ms=round(ms, 4), gpts=round(n ** 3 / ms / 1e6, 2),
-                             gbps=round(gbps, 1), stream_frac=round(gbps / copy_gbps, 3)))
+            ent = dict(name=name, ms=round(ms, 4), gpts=round(n ** 3 / ms / 1e6, 2),
+                       gbps=round(gbps, 1), stream_frac=round(gbps / copy_gbps, 3),
+                       hbm_frac=round(gbps / HBM_PEAK_GBPS, 3))
+            ent.update(slab_errors3d(dst, 2, n, wl.taps3d("face_edge" if shape == L.FACE_EDGE else "star", r),
+                                     coef, r, dt))
+            kern.append(ent)
         # multi-sweep runs launch two sweeps at a time (one HBM read + write per
         # two sweeps): per-sweep rates above the single-sweep bandwidth bound;
         # effective_gbps counts the bytes of single sweeps
@@ -503,9 +612,36 @@ def side_3d(stream, copy_gbps):
     return out
 
 
+def slab_errors3d(got, seed: int, n: int, taps, coef, r: int, dt) -> dict:
+    """Error of a one-sweep n^3 result against the oracle (interior-only gold, zero
+    boundary) on three 8-plane z-slabs; the input planes are regenerated on the
+    host from the device fill's stream."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle
+
+    npdt = np.float32 if dt == torch.float32 else np.float64
+    mx, sq, cnt, gmax = 0.0, 0.0, 0, 0.0
+    for z0 in (0, n // 2 - 4, n - 8):
+        lo, hi = max(0, z0 - r), min(n, z0 + 8 + r)
+        x = oracle.uniform(seed, (hi - lo) * n * n, 0.0, 1.0, npdt, offset=lo * n * n).reshape(hi - lo, n, n)
+        want = oracle.stencil3d_taps(x.astype(np.float64), taps, coef, r)
+        a = z0 - lo
+        w = want[a:a + 8]
+        g = got[z0:z0 + 8].double().cpu().numpy()  # planes a..a+7 sit r planes inside the slab
+        d = np.abs(g - w)
+        mx = max(mx, float(d.max()))
+        sq += float((d * d).sum())
+        cnt += d.size
+        gmax = max(gmax, float(np.abs(w).max()))
+    return dict(max_abs_err=mx, rms_err=(sq / cnt) ** 0.5, max_rel_err_normwise=mx / gmax,
+                against="FP64 oracle on z-slabs 0-7, n/2-4..n/2+3, n-8..n-1")
+
+
 def side_conv(stream, copy_gbps, tune=True):
     """8192^2 image U[0,1), K x K taps U[-1,1)/K^2: FP32 plain vs BF16 / FP16 storage
-    (FP32 accumulation); errors against the FP64 GPU path on the same values."""
+    (FP32 accumulation); errors against the FP64 oracle on row bands."""
     import torch
 
     from ssam_amd import _lib as L
@@ -518,8 +654,6 @@ def side_conv(stream, copy_gbps, tune=True):
     rows = []
     for k in (3, 5, 7, 9, 15, 19):
         f = wl.conv_taps(k, seed=50 + k)
-        ref = torch.empty_like(x64)
-        api.stencil2d(x64, ref, f, shape=L.DENSE, stream=stream)
         row = dict(K=k)
         for label, dt in (("fp64", torch.float64), ("fp32", torch.float32), ("bf16", torch.bfloat16),
                           ("fp16", torch.float16)):
@@ -528,17 +662,20 @@ def side_conv(stream, copy_gbps, tune=True):
             if tune:
                 api.tune2d(src, dst, f, shape=L.DENSE, stream=stream)
             ms = _time(lambda: api.stencil2d(src, dst, f, shape=L.DENSE, stream=stream), stream)
-            ent = dict(ms=round(ms, 4), gpts=round(n * n / ms / 1e6, 2))
-            if dt != torch.float64:
-                d = (dst.to(torch.float64) - ref).abs()
-                ent["max_abs_err"] = float(d.max())
-                ent["rms_err"] = float(torch.sqrt((d * d).mean()))
-                ent["max_rel_err_normwise"] = float(d.max() / ref.abs().max())
+            flops = n * n * (2 * k * k - 1)
+            ent = dict(ms=round(ms, 4), gpts=round(n * n / ms / 1e6, 2),
+                       tflops=round(flops / ms / 1e9, 2))
+            if dt in (torch.float32, torch.float64):
+                ent["roofline"] = roof(flops, 2 * src.element_size() * n * n, ms,
+                                       FP32_PEAK_TFLOPS if dt == torch.float32 else FP64_PEAK_TFLOPS)
+            api.stencil2d(src, dst, f, shape=L.DENSE, stream=stream)
+            torch.cuda.synchronize()
+            ent.update(band_errors(dst, 3, 0.0, 1.0, f))
             row[label] = ent
             del dst
         row["bf16_speedup_vs_fp32"] = round(row["fp32"]["ms"] / row["bf16"]["ms"], 3)
+        row["fp16_speedup_vs_fp32"] = round(row["fp32"]["ms"] / row["fp16"]["ms"], 3)
         rows.append(row)
-        del ref
     return rows
 
 
@@ -567,6 +704,8 @@ def side_batch(stream):
         bd = torch.empty_like(big)
         for (sh, r), code in shapes.items():
             f = wl.normalised_taps(sh, r, 60 + r)
+            # the best single-grid rate: autotuned like the batch
+            api.tune2d(big, bd, f, shape=code, stream=stream)
             ms = _time(lambda: api.stencil2d(big, bd, f, shape=code, stream=stream), stream)
             ref[(lab, sh, r)] = N * N / ms / 1e6
         del big, bd
@@ -619,6 +758,9 @@ def side_batch(stream):
         torch.cuda.synchronize()
         ms = _time(lambda: plan.run(stream=stream), stream, reps=5)
         ms_eager = _time(lambda: plan.run(eager=True, stream=stream), stream, reps=3)
+        plan.run(stream=stream)
+        torch.cuda.synchronize()
+        errs = batch_errors(g2, g3, items)
         out[lab] = dict(ms=round(ms, 4), ms_eager=round(ms_eager, 4), point_updates=pu,
                         gpts=round(pu / ms / 1e6, 2), tiles_2d=t2, tiles_3d=t3,
                         points_2d=pts2, point_updates_3d=pts3,
@@ -626,12 +768,47 @@ def side_batch(stream):
                         efficiency_vs_single_grid=round(exp_ms / ms, 3),
                         part_ms=parts, single_grid_equivalent_2d_ms=round(exp2, 4),
                         single_grid_equivalent_3d_ms=round(exp_ms - exp2, 4),
-                        storage_2d=lab, storage_3d="fp32", tiling=tiling)
+                        storage_2d=lab, storage_3d="fp32", tiling=tiling, error_vs_fp64_oracle=errs)
         plan.close()
         del keep, items
         torch.cuda.empty_cache()
     out["single_grid_gpts"] = {f"{k[0]}_{k[1]}{k[2]}": round(v, 1) for k, v in ref.items()}
     out["single_grid_gpts"]["fp32_3d7pt_x10"] = round(ref3, 1)
+    return out
+
+
+def batch_errors(g2, g3, items) -> dict:
+    """Max error of sampled batch items against the FP64 oracle: the six smallest
+    2D grids of each shape (every point) and the four smallest 3D grids (every
+    point, all 10 sweeps)."""
+    import numpy as np
+
+    from oracle import oracle
+    from ssam_amd import workloads as wl
+
+    out = {}
+    for shape, r in (("star", 1), ("box", 2), ("box", 5)):
+        ks = sorted((g["w"] * g["h"], k) for k, g in enumerate(g2) if (g["shape"], g["r"]) == (shape, r))[:6]
+        mx, gm = 0.0, 0.0
+        for _, k in ks:
+            g = g2[k]
+            x = oracle.uniform(1000 + k, g["w"] * g["h"], -1.0, 1.0, np.float64).reshape(g["h"], g["w"])
+            want = oracle.conv2d(x, wl.normalised_taps(shape, r, 60 + r))
+            got = items[k]["dst"].double().cpu().numpy()
+            mx, gm = max(mx, float(np.abs(got - want).max())), max(gm, float(np.abs(want).max()))
+        out[f"{shape}{2 * r + 1}"] = dict(items=len(ks), max_abs_err=mx, max_rel_err_normwise=mx / gm)
+    ks = sorted((g["nx"] * g["ny"] * g["nz"], k) for k, g in enumerate(g3))[:4]
+    mx, gm = 0.0, 0.0
+    for _, k in ks:
+        g = g3[k]
+        x = oracle.uniform(5000 + k, g["nx"] * g["ny"] * g["nz"], 0.0, 1.0, np.float32).reshape(
+            g["nz"], g["ny"], g["nx"]).astype(np.float64)
+        for _ in range(g["steps"]):
+            x = oracle.stencil3d_taps(x, wl.taps3d("star", 1), wl.REF3D_7PT, 1)
+        got = items[len(g2) + k]["dst"].double().cpu().numpy()
+        mx, gm = max(mx, float(np.abs(got - x).max())), max(gm, float(np.abs(x).max()))
+    out["3d_7pt_x10"] = dict(items=len(ks), max_abs_err=mx, max_rel_err_normwise=mx / gm)
+    out["against"] = "FP64 oracle, every point of the sampled items"
     return out
 
 
@@ -643,7 +820,7 @@ def cpu_baseline(suite, taps):
     from oracle import oracle
     from ssam_amd import workloads as wl
 
-    rows = 64
+    rows = 1024
     x = wl.uniform(0, rows * N, -1.0, 1.0).astype(np.float32).reshape(rows, N)
     upd, sec = 0, 0.0
     for name, shape, r, t in suite:
