@@ -46,6 +46,8 @@ SOURCES = [
     "stencil2d_box_bf16.hip",
     "stencil2d_box_f16.hip",
     "stencil2d_even_f32.hip",
+    "stencil2d_lds_box_f32.hip",
+    "stencil2d_lds_even_f32.hip",
     "stencil2d_even_f64.hip",
     "stencil2d_kahan_f32.hip",
     "stencil2d_kahan_bf16.hip",

@@ -809,6 +809,9 @@ int ssam_tune_stencil2d(const ssam_stencil2d_desc* d, const void* src, void* dst
     if ((d->storage == SSAM_BF16 || d->storage == SSAM_F16) && d->shape != SSAM_SHAPE_STAR &&
         !(d->flags & SSAM_FLAG_COMPENSATED) && d->steps == 1)
         form_c.push_back(kFormMfma);
+    // the LDS-tap form for FP32 boxes and dense filters
+    if (d->storage == SSAM_F32 && (kshape == kBox || kshape == kBoxEven) && R >= 2 && d->steps == 1)
+        form_c.push_back(kFormLds);
     for (int form : form_c)
     for (int tc : tch_c) {
         for (int wps : wps_c) {
@@ -883,7 +886,7 @@ int ssam_tune_load(const char* path) {
         double ms;
         const int nf = std::sscanf(line, "%d %d %d %d %lld %lld %d %d %lf %d", &sto, &sh, &r, &steps, &w,
                                    &h, &wps, &tc, &ms, &form);
-        if (nf >= 9 && wps > 0 && wps <= 64 && tc >= 0 && form >= kFormAuto && form <= kFormMfma) {
+        if (nf >= 9 && wps > 0 && wps <= 64 && tc >= 0 && form >= kFormAuto && form <= kFormLds) {
             g_tune[TuneKey{sto, sh, r, steps, w, h}] = TuneVal{wps, tc, static_cast<float>(ms), form};
             ++n;
         }

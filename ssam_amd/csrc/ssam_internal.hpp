@@ -37,8 +37,9 @@ struct Req2D {
     int form;            // Form2D: kernel family (auto picks the matrix cores where they apply)
 };
 
-// 2D kernel family: register-cache VALU kernels or the matrix-core (MFMA) form.
-enum Form2D : int { kFormAuto = 0, kFormValu = 1, kFormMfma = 2 };
+// 2D kernel family: register-cache VALU kernels, the matrix-core (MFMA) form, or
+// the LDS-tap VALU form (FP32 boxes and dense filters; taps staged in LDS).
+enum Form2D : int { kFormAuto = 0, kFormValu = 1, kFormMfma = 2, kFormLds = 3 };
 
 template <typename S> int dispatch2d(const Req2D& rq, hipStream_t st);
 

@@ -3,6 +3,7 @@
 // tiles and converts the taps to the accumulation type.
 #pragma once
 #include "stencil2d.hpp"
+#include "stencil2d_lds.hpp"
 #include "stencil2d_mfma.hpp"
 #include <type_traits>
 #include <vector>
@@ -10,6 +11,40 @@
 #include "ssam_internal.hpp"
 
 namespace ssam {
+
+// Waves the device holds at once for a kernel of 4-wave workgroups: resident
+// workgroups per CU (the occupancy API, which accounts for the kernel's VGPRs,
+// SGPRs and LDS) x 4 x CUs; cached per kernel by the caller.
+inline long long resident_waves(const void* kernel) {
+    int dev = 0, cus = 0, blocks = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kWave * kWavesPerBlock2D, 0) !=
+            hipSuccess || blocks < 1)
+        return 0;
+    return static_cast<long long>(blocks) * kWavesPerBlock2D * cus;
+}
+
+// Output rows per tile for a grid of `nstrips` column strips.  The tile count
+// aims at waves_per_simd * 1024 waves, rounded to a whole number of device-wide
+// rounds of `slots` resident waves (so no round holds a handful of stragglers:
+// 4130 tiles on 4096 slots ran a second round for 34 of them), and never below
+// min_seg rows per tile (the 2*T*r halo rows per tile stay a small fraction).
+inline int tile_rows(int64_t height, int nstrips, int waves_per_simd, long long slots, int min_seg) {
+    long long target = static_cast<long long>(waves_per_simd) * 1024;
+    if (slots > 0) {
+        long long rounds = (target + slots / 2) / slots;
+        if (rounds < 1) rounds = 1;
+        target = rounds * slots;
+    }
+    long long nsegs = target / nstrips;  // rounded down: at most `target` tiles
+    if (nsegs < 1) nsegs = 1;
+    long long seg = (height + nsegs - 1) / nsegs;
+    if (seg < min_seg) seg = min_seg;
+    if (seg > height) seg = height;
+    return static_cast<int>(seg);
+}
 
 // The ssam_dtype of a storage type.
 template <typename S> constexpr int storage_of() {
@@ -90,15 +125,13 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
                (reinterpret_cast<uintptr_t>(rq.dst) % sizeof(SO) == 0);
     // rows per tile: enough tiles for ~target waves per SIMD, but at least a few
     // multiples of the halo so the 2*T*r re-read rows stay a small fraction
+    auto kernel = ssam2d_kernel<S, R, SHAPE, V, T, PF, RING, U, WPE, SO>;
+    static const long long slots = resident_waves(reinterpret_cast<const void*>(kernel));
     int seg = rq.seg_rows;
     if (seg <= 0) {
-        const long long target_waves = static_cast<long long>(rq.waves_per_simd) * 1024;
-        long long nsegs = (target_waves + p.nstrips - 1) / p.nstrips;
-        if (nsegs < 1) nsegs = 1;
-        seg = static_cast<int>((rq.height + nsegs - 1) / nsegs);
         // (8 rows at least: small grids need many short tiles to fill the device)
         const int min_seg = 8 * T * R > 8 ? 8 * T * R : 8;
-        if (seg < min_seg) seg = min_seg;
+        seg = tile_rows(rq.height, p.nstrips, rq.waves_per_simd, slots, min_seg);
     }
     if (seg > p.height) seg = p.height;
     p.seg_rows = seg;
@@ -108,11 +141,76 @@ static int launch2d_instance(const Req2D& rq, hipStream_t stream) {
     p.ntiles = static_cast<int>(ntiles);
     p.nblocks = (p.ntiles + kWavesPerBlock2D - 1) / kWavesPerBlock2D;
 
-    hipLaunchKernelGGL((ssam2d_kernel<S, R, SHAPE, V, T, PF, RING, U, WPE, SO>), dim3(p.nblocks),
-                       dim3(kWave * kWavesPerBlock2D), 0, stream, tp, static_cast<const S*>(rq.src),
-                       static_cast<SO*>(rq.dst), p);
+    hipLaunchKernelGGL(kernel, dim3(p.nblocks), dim3(kWave * kWavesPerBlock2D), 0, stream, tp,
+                       static_cast<const S*>(rq.src), static_cast<SO*>(rq.dst), p);
     return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
 }
+
+// ---------------------------------------------------------------- LDS-tap form
+// Compile-time shape of the LDS-tap kernel per radius (FP32): 4 columns per lane
+// up to radius 9 (2 beyond, where the input ring of 2r+1+PF+M-1 rows would not fit
+// beside the partial sums), 2 output rows per block, rows prefetched 3 ahead; the
+// register budget (waves per SIMD the allocator must leave room for) is 4 up to
+// radius 5 and 3 beyond.
+template <int R> struct LdsShape {
+    static constexpr int V = R <= 9 ? 4 : 2;
+    static constexpr int M = 2;
+    static constexpr int PF = 3;
+    static constexpr int WPE = R <= 5 ? 4 : 3;
+};
+
+template <typename S, int R, int SHAPE>
+static int launch2d_lds_instance(const Req2D& rq, hipStream_t stream) {
+    using A = typename Storage<S>::acc;
+    using LS = LdsShape<R>;
+    using G = Geom2D<R, LS::V, 1>;
+    using KT = KernargTaps<A, R, SHAPE>;
+    static_assert(KT::fits, "FP32 taps always fit the kernel-argument segment");
+    Taps<A, KT::count> tp{};
+    Params2D p;
+    p.taps = rq.dev_taps;
+    if (!rq.dev_taps)
+        for (int i = 0; i < KT::n; ++i) tp.w[i] = static_cast<A>(rq.taps[i]);
+    p.width = static_cast<int>(rq.width);
+    p.height = static_cast<int>(rq.height);
+    p.pitch = rq.pitch;
+    p.pitch_out = rq.pitch_out;
+    p.nstrips = static_cast<int>((rq.width + G::OW - 1) / G::OW);
+    p.boundary = rq.boundary;
+    p.vec_ok = (reinterpret_cast<uintptr_t>(rq.src) % sizeof(S) == 0) &&
+               (reinterpret_cast<uintptr_t>(rq.dst) % sizeof(S) == 0);
+    auto kernel = ssam2d_lds_kernel<S, R, SHAPE, LS::V, LS::M, LS::PF, LS::WPE>;
+    static const long long slots = resident_waves(reinterpret_cast<const void*>(kernel));
+    int seg = rq.seg_rows > 0 ? rq.seg_rows
+                              : tile_rows(rq.height, p.nstrips, rq.waves_per_simd, slots, 8 * R > 8 ? 8 * R : 8);
+    if (seg > p.height) seg = p.height;
+    p.seg_rows = seg;
+    const long long ntiles = static_cast<long long>(p.nstrips) * ((p.height + seg - 1) / seg);
+    if (ntiles > (1LL << 30)) return SSAM_E_UNSUPPORTED;
+    p.ntiles = static_cast<int>(ntiles);
+    p.nblocks = (p.ntiles + kWavesPerBlock2D - 1) / kWavesPerBlock2D;
+    hipLaunchKernelGGL(kernel, dim3(p.nblocks), dim3(kWave * kWavesPerBlock2D), 0, stream, tp,
+                       static_cast<const S*>(rq.src), static_cast<S*>(rq.dst), p);
+    return hipGetLastError() == hipSuccess ? SSAM_OK : SSAM_E_HIP;
+}
+
+// FP32 boxes and dense filters of radius 2..14 on the LDS-tap form; each shape
+// has its own compile unit (stencil2d_lds_<shape>_f32.hip).
+template <typename S, int SHAPE> int dispatch_lds(const Req2D& rq, hipStream_t st);
+#ifdef SSAM_2D_LDS_UNIT
+template <typename S, int SHAPE> int dispatch_lds(const Req2D& rq, hipStream_t st) {
+    switch (rq.radius) {
+#define SSAM_R(r) \
+    case r:       \
+        return launch2d_lds_instance<S, r, SHAPE>(rq, st);
+        SSAM_R(2) SSAM_R(3) SSAM_R(4) SSAM_R(5) SSAM_R(6) SSAM_R(7) SSAM_R(8) SSAM_R(9) SSAM_R(10)
+        SSAM_R(11) SSAM_R(12) SSAM_R(13) SSAM_R(14)
+#undef SSAM_R
+        default:
+            return SSAM_E_UNSUPPORTED;
+    }
+}
+#endif
 
 // radius is a compile-time instance parameter; these expand the runtime switch.
 // Stars go to radius 6, compensated boxes to 9 (19 x 19), boxes and even filters
@@ -250,6 +348,14 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
                 return dispatch_radius<S, kStar, V, 1>(rq, st);
             }
         }
+        // FP32 boxes and dense filters of radius >= 2: the LDS-tap form when asked
+        // for, and by default from radius 3 (where the register-cache kernel's
+        // resident taps leave too few waves per SIMD for the double FMA rate)
+        if constexpr (std::is_same<S, float>::value) {
+            if ((rq.shape == kBox || rq.shape == kBoxEven) && rq.radius >= 2 &&
+                (rq.form == kFormLds || (rq.form == kFormAuto && rq.radius >= 3)))
+                return rq.shape == kBoxEven ? dispatch_lds<S, kBoxEven>(rq, st) : dispatch_lds<S, kBox>(rq, st);
+        }
         // FP64 storage accumulates in FP64: compensation changes nothing there
         if constexpr (sizeof(S) != 8) {
             if (rq.shape == kBoxKahan) return dispatch_shape<S, kBoxKahan>(rq, st);
@@ -274,3 +380,6 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
 
 #define SSAM_INSTANTIATE_2D_SHAPE(S, SHAPE) \
     template int ssam::dispatch_shape<S, SHAPE>(const ssam::Req2D&, hipStream_t);
+
+#define SSAM_INSTANTIATE_2D_LDS(S, SHAPE) \
+    template int ssam::dispatch_lds<S, SHAPE>(const ssam::Req2D&, hipStream_t);
