@@ -563,7 +563,9 @@ def side_config0(stream, copy_gbps):
 def side_3d(stream, copy_gbps):
     """512^3: 7-pt and 13-pt star sweeps (reference coefficients), the reference
     19-pt 'poisson', 10 sweeps of the 7-pt star, and 100 Jacobi sweeps of the
-    19-point Laplacian with the FP64 residual of the last sweep -- FP32 and FP64."""
+    19-point Laplacian with the FP64 residual of the last sweep -- FP32 and FP64,
+    and BF16 / FP16 storage (FP32 accumulation, FP32 running state between
+    launches in a work area of two FP32 grids)."""
     import torch
 
     from ssam_amd import _lib as L
@@ -572,11 +574,19 @@ def side_3d(stream, copy_gbps):
 
     n = 512
     out = {}
-    for label, dt, es in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
-        src = torch.empty((n, n, n), dtype=dt, device="cuda")
-        api.fill_uniform(src, 2, 0.0, 1.0)
+    for label, dt, es in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8),
+                          ("bf16", torch.bfloat16, 2), ("fp16", torch.float16, 2)):
+        if es == 2:
+            # the FP32 grid's values, rounded to the storage type
+            src = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
+            api.fill_uniform(src, 2, 0.0, 1.0)
+            src = src.to(dt)
+            work = torch.empty(2 * n ** 3, dtype=torch.float32, device="cuda")
+        else:
+            src = torch.empty((n, n, n), dtype=dt, device="cuda")
+            api.fill_uniform(src, 2, 0.0, 1.0)
+            work = torch.empty_like(src)
         dst = torch.empty_like(src)
-        work = torch.empty_like(src)
         kern = []
         for name, shape, r, coef in (("7pt", L.STAR, 1, wl.REF3D_7PT), ("13pt", L.STAR, 2, wl.REF3D_13PT),
                                      ("poisson19", L.FACE_EDGE, 1, wl.REF3D_POISSON)):
@@ -607,8 +617,13 @@ def side_3d(stream, copy_gbps):
         kern.append(dict(name="jacobi19_x100", ms_per_step=round(ms, 4),
                          gpts=round(n ** 3 / ms / 1e6, 2), effective_gbps=round(gbps, 1),
                          sweeps_per_launch=2, residual_l2_fp64=res))
+        if es == 2:
+            for a, b in zip(kern, out["fp32"]):
+                ka = "ms" if "ms" in a else "ms_per_step"
+                a["speedup_vs_fp32"] = round(b[ka] / a[ka], 3)
         out[label] = kern
         del src, dst, work
+        torch.cuda.empty_cache()
     return out
 
 
@@ -710,13 +725,20 @@ def side_batch(stream):
             ref[(lab, sh, r)] = N * N / ms / 1e6
         del big, bd
     n3 = 512
-    b3 = torch.empty((n3, n3, n3), dtype=torch.float32, device="cuda")
-    api.fill_uniform(b3, 6, 0.0, 1.0)
-    d3, w3 = torch.empty_like(b3), torch.empty_like(b3)
-    ms = _time(lambda: api.stencil3d(b3, d3, wl.REF3D_7PT, shape=L.STAR, radius=1, steps=10, work=w3,
-                                     stream=stream), stream, reps=3)
-    ref3 = n3 ** 3 * 10 / ms / 1e6
-    del b3, d3, w3
+    ref3 = {}
+    for dt, lab in ((torch.float32, "fp32"), (torch.bfloat16, "bf16")):
+        b3 = torch.empty((n3, n3, n3), dtype=torch.float32, device="cuda")
+        api.fill_uniform(b3, 6, 0.0, 1.0)
+        b3 = b3.to(dt)
+        d3 = torch.empty_like(b3)
+        # (BF16: the running state between launches stays FP32, two FP32 grids)
+        w3 = torch.empty_like(b3) if dt == torch.float32 else torch.empty(2 * n3 ** 3, dtype=torch.float32,
+                                                                          device="cuda")
+        ms = _time(lambda: api.stencil3d(b3, d3, wl.REF3D_7PT, shape=L.STAR, radius=1, steps=10, work=w3,
+                                         stream=stream), stream, reps=3)
+        ref3[lab] = n3 ** 3 * 10 / ms / 1e6
+        del b3, d3, w3
+        torch.cuda.empty_cache()
     for dt, lab in ((torch.float32, "fp32"), (torch.bfloat16, "bf16")):
         items, keep = [], []
         exp_ms = 0.0
@@ -734,13 +756,17 @@ def side_batch(stream):
         for k, g in enumerate(g3):
             src = torch.empty((g["nz"], g["ny"], g["nx"]), dtype=torch.float32, device="cuda")
             api.fill_uniform(src, 5000 + k, 0.0, 1.0)
-            dst, work = torch.empty_like(src), torch.empty_like(src)
+            src = src.to(dt)
+            dst = torch.empty_like(src)
+            # BF16 items keep their state between passes in FP32 (two FP32 grids)
+            work = torch.empty_like(src) if dt == torch.float32 else \
+                torch.empty(2 * src.numel(), dtype=torch.float32, device="cuda")
             keep += [src, dst, work]
             items.append(dict(ndim=3, src=src, dst=dst, work=work, shape=L.STAR, radius=1,
                               steps=g["steps"], coeffs=wl.REF3D_7PT))
             n = g["nx"] * g["ny"] * g["nz"] * g["steps"]
             pts3 += n
-            exp_ms += n / (ref3 * 1e6)
+            exp_ms += n / (ref3[lab] * 1e6)
         # the 2D and 3D halves alone, to attribute the time
         parts = {}
         for part, sel in (("2d_only", lambda it: it["ndim"] == 2), ("3d_only", lambda it: it["ndim"] == 3)):
@@ -768,12 +794,13 @@ def side_batch(stream):
                         efficiency_vs_single_grid=round(exp_ms / ms, 3),
                         part_ms=parts, single_grid_equivalent_2d_ms=round(exp2, 4),
                         single_grid_equivalent_3d_ms=round(exp_ms - exp2, 4),
-                        storage_2d=lab, storage_3d="fp32", tiling=tiling, error_vs_fp64_oracle=errs)
+                        storage_2d=lab, storage_3d=lab, tiling=tiling, error_vs_fp64_oracle=errs)
         plan.close()
         del keep, items
         torch.cuda.empty_cache()
     out["single_grid_gpts"] = {f"{k[0]}_{k[1]}{k[2]}": round(v, 1) for k, v in ref.items()}
-    out["single_grid_gpts"]["fp32_3d7pt_x10"] = round(ref3, 1)
+    for lab, v in ref3.items():
+        out["single_grid_gpts"][f"{lab}_3d7pt_x10"] = round(v, 1)
     return out
 
 

@@ -91,12 +91,14 @@ typedef struct {
 } ssam_stencil2d_desc;
 
 typedef struct {
-    int32_t storage;      /* ssam_dtype (F32 or F64) */
+    int32_t storage;      /* ssam_dtype: F32, F64, BF16 or F16 */
     int32_t shape;        /* STAR (radius 1 or 2) or FACE_EDGE */
     int32_t radius;       /* 1 or 2 for STAR, 1 for FACE_EDGE */
     int32_t steps;        /* sweeps (>= 1) */
     int32_t boundary;     /* SSAM_BOUNDARY_DIRICHLET0 */
-    uint32_t flags;
+    uint32_t flags;       /* SSAM_FLAG_WORK_FP32 (BF16 / F16 with steps > 1: `work`
+                             holds 2 * nx*ny*nz FP32 values); other bits:
+                             SSAM_E_INVALID */
     const double* coeffs; /* HOST memory.  STAR r: 6r+1 taps ordered
                              c, x-1, x+1, y-1, y+1, z-1, z+1, x-2, x+2, y-2, y+2, z-2, z+2.
                              FACE_EDGE: 19 taps ordered c, 6 faces (as STAR r=1), then
@@ -177,9 +179,12 @@ int ssam_conv2d_f64(const double* src, double* dst, int width, int width_stride,
 
 /* ---------------------------------------------------------------- 3D
  * `steps` sweeps of a 3D stencil on a dense nx*ny*nz grid (index z*ny*nx + y*nx + x,
- * tmp3d7pt-v2.cu:107), F32 or F64.  Every output point is written: the outer r
- * layers are 0 (SSAM_BOUNDARY_DIRICHLET0), the interior gets the stencil of the
- * input.  steps > 1 ping-pongs through `work` (same size; may be NULL otherwise).
+ * tmp3d7pt-v2.cu:107).  Every output point is written: the outer r layers are 0
+ * (SSAM_BOUNDARY_DIRICHLET0), the interior gets the stencil of the input.
+ * steps > 1 ping-pongs through `work` (may be NULL otherwise): F32 / F64 -- the
+ * grid's size; BF16 / F16 storage (FP32 accumulation) -- 2 * nx*ny*nz FP32 values
+ * with SSAM_FLAG_WORK_FP32 set, where the running state stays FP32 between
+ * launches (the grid is read and written in the storage type once each).
  * Replaces kernel3d_3d7pt (tmp3d7pt-v2.cu:62-64), kernel3d_3d13pt
  * (tmp3d13pt.cu:74-76), kernel3d_poisson (tmpPoisson.cu:86-88). */
 int ssam_stencil3d(const ssam_stencil3d_desc* desc, const void* src, void* dst, int64_t nx,
@@ -198,8 +203,10 @@ int ssam_poisson_f64(const double* src, double* dst, int n, int iters, double* w
 
 /* 19-point Jacobi sweeps for the 3D Laplace problem (this library's operator; the
  * reference's "poisson" grows without bound when iterated):
- *   u'(c) = (2*sum_faces u + sum_edges u) / 24, zero Dirichlet boundary,
- * running state in the storage type (F32 or F64, never BF16).  If residual_l2 is
+ *   u'(c) = (2*sum_faces u + sum_edges u) / 24, zero Dirichlet boundary.
+ * F32 / F64 storage: running state in the storage type, `work` the grid's size;
+ * BF16 / F16 storage: the running state stays FP32 and `work` holds
+ * 2 * nx*ny*nz FP32 values (steps > 1).  If residual_l2 is
  * not NULL it receives the FP64 L2 norm of r = -A u of the input of the LAST
  * sweep, A u = (24 u - 2 faces - edges)/6, reduced on the GPU in a fixed order;
  * this entry then synchronises `stream`. */
@@ -222,8 +229,9 @@ int ssam_jacobi19_noalloc(int storage, const void* src, void* dst, int64_t nx, i
  * Many independent grids of uneven sizes and mixed shapes.  2D items of one
  * storage type whose shape is a 5-point star or a 3x3 / 5x5 / 11x11 box run as
  * one launch per stencil kind (or one for all kinds); 3D items (7-point star or
- * 19-point face+edge, F32/F64, under 2 GiB) run one launch per pass of one or two
- * sweeps for each (storage, steps, shape); any other item gets its own launch.
+ * 19-point face+edge, any storage, under 2 GiB) run one launch per pass of one or
+ * two sweeps for each (storage, steps, shape) -- BF16 / F16 items two sweeps per
+ * pass with their state FP32 between passes; any other item gets its own launch.
  * Each launch is an ordinary grid with one tile per wave over a table sorted by
  * decreasing tile cost (default), or a persistent grid taking tiles from 8 device
  * work queues, one per XCD (neighbouring tiles of a grid share that XCD's L2),
@@ -235,7 +243,7 @@ int ssam_jacobi19_noalloc(int storage, const void* src, void* dst, int64_t nx, i
  * reference. */
 typedef struct {
     int32_t ndim;          /* 2 or 3 */
-    int32_t storage;       /* ssam_dtype (3D: F32 or F64) */
+    int32_t storage;       /* ssam_dtype */
     int32_t shape;         /* 2D: STAR or BOX; 3D: STAR or FACE_EDGE */
     int32_t radius;        /* filter radius r */
     int32_t steps;         /* 2D: 1 in the persistent launch; 3D: sweeps */
@@ -244,7 +252,8 @@ typedef struct {
     int64_t pitch;         /* 2D row pitch in elements; 3D grids are dense */
     const void* src;
     void* dst;
-    void* work;            /* 3D with steps > 1: ping-pong buffer of the same size */
+    void* work;            /* 3D with steps > 1: ping-pong buffer of the same size
+                              (BF16 / F16: 2 * nx*ny*nz FP32 values, the state) */
     const double* coeffs;  /* HOST: 2D (2r+1)^2 row-major; 3D descriptor tap order */
 } ssam_batch_item;
 

@@ -209,9 +209,21 @@ def _check_3d(src, dst) -> tuple[int, int, int]:
     return nx, ny, nz
 
 
+def _work3d_flags(src: torch.Tensor, work: Optional[torch.Tensor]) -> int:
+    """BF16 / FP16 grids with a float32 `work` of 2 * nx * ny * nz values: the
+    running state between launches stays FP32 there (SSAM_FLAG_WORK_FP32)."""
+    if work is not None and src.element_size() == 2 and work.dtype == torch.float32:
+        if work.numel() < 2 * src.numel() or not work.is_contiguous():
+            raise ValueError("a float32 work area for BF16/FP16 grids holds 2 * nx * ny * nz values")
+        return L.FLAG_WORK_FP32
+    return 0
+
+
 def stencil3d(src, dst, coeffs, shape: int = L.STAR, radius: int = 1, steps: int = 1,
               work=None, stream=None) -> None:
-    """ssam_stencil3d: `steps` sweeps, zero Dirichlet output boundary."""
+    """ssam_stencil3d: `steps` sweeps, zero Dirichlet output boundary.  BF16 / FP16
+    grids of more than one launch need a float32 `work` of 2 * nx * ny * nz values
+    (their running state stays FP32)."""
     nx, ny, nz = _check_3d(src, dst)
     c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
     d = L.Stencil3DDesc()
@@ -220,7 +232,7 @@ def stencil3d(src, dst, coeffs, shape: int = L.STAR, radius: int = 1, steps: int
     d.radius = radius
     d.steps = steps
     d.boundary = L.DIRICHLET0
-    d.flags = 0
+    d.flags = _work3d_flags(src, work)
     d.coeffs = c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     check(L.lib().ssam_stencil3d(ctypes.byref(d), _dev(src, "src"), _dev(dst, "dst"), nx, ny, nz,
                                  _dev(work, "work"), _stream(stream)), "ssam_stencil3d")
@@ -239,8 +251,13 @@ def ref3d(name: str, src, dst, iters: int = 1, work=None, stream=None) -> None:
 
 def jacobi19(src, dst, steps: int, work=None, residual: bool = True, stream=None) -> Optional[float]:
     """19-point Jacobi sweeps (zero Dirichlet); returns the FP64 residual L2 norm of
-    the last sweep's input (synchronises the stream) when `residual`."""
+    the last sweep's input (synchronises the stream) when `residual`.  BF16 / FP16
+    grids: `work` is float32 with 2 * nx * ny * nz values (the FP32 running state)."""
     nx, ny, nz = _check_3d(src, dst)
+    if src.element_size() == 2 and work is not None:
+        _work3d_flags(src, work)
+        if work.dtype != torch.float32:
+            raise ValueError("BF16/FP16 Jacobi runs keep their state in a float32 work area")
     r = ctypes.c_double(0.0)
     check(L.lib().ssam_jacobi19(dtype_code(src.dtype), _dev(src, "src"), _dev(dst, "dst"), nx, ny,
                                 nz, steps, _dev(work, "work"),
@@ -269,9 +286,10 @@ class BatchPlan:
     """Many grids in one plan (ssam_batch_plan_create / ssam_batch_run).
 
     items: dicts with keys ndim (2/3), src, dst, coeffs, shape, radius, and for 3D
-    steps and work.  2D tensors are (h, w) with contiguous rows; 3D tensors are
-    contiguous (nz, ny, nx).  The plan binds the device pointers; keep the tensors
-    alive while it exists."""
+    steps and work (BF16 / FP16 3D items of more than one pass: float32 with
+    2 * nx * ny * nz values, their FP32 running state).  2D tensors are (h, w) with
+    contiguous rows; 3D tensors are contiguous (nz, ny, nx).  The plan binds the
+    device pointers; keep the tensors alive while it exists."""
 
     def __init__(self, items):
         self._keep = []

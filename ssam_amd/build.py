@@ -53,10 +53,15 @@ SOURCES = [
     "stencil2d_kahan_bf16.hip",
     "stencil2d_kahan_f16.hip",
     "stencil3d.hip",
+    "stencil3d_f32f64.hip",
+    "stencil3d_bf16.hip",
+    "stencil3d_f16.hip",
     "batch.hip",
     "batch2d.hip",
     "batch3d.hip",
     "batch3d_t2.hip",
+    "batch3d_t2_bf16.hip",
+    "batch3d_t2_f16.hip",
     "microbench.hip",
 ]
 

@@ -550,15 +550,19 @@ struct ResidualArea {
     double* out_dev;   // device norm (fixed area only)
 };
 
+// `steps` sweeps of one 3D shape.  F32 / F64 chains of launches ping-pong
+// through `work` (the grid's size).  BF16 / F16 chains keep their running state
+// in FP32: `work` then holds two dense FP32 grids (work_fp32 says the caller
+// sized it so; without it such a chain is SSAM_E_INVALID), the first launch reads
+// the storage type and the last writes it.
 int stencil3d_run(int storage, int shape, int radius, const double* taps, int steps,
                   const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz, void* work,
-                  double* residual_l2, double res_scale, hipStream_t st,
+                  bool work_fp32, double* residual_l2, double res_scale, hipStream_t st,
                   const ResidualArea* fixed = nullptr) {
     if (!src || !dst || !taps) return SSAM_E_INVALID;
-    if (storage != SSAM_F32 && storage != SSAM_F64) {
-        if (storage == SSAM_BF16 || storage == SSAM_F16) return SSAM_E_UNSUPPORTED;
+    if (storage != SSAM_F32 && storage != SSAM_F64 && storage != SSAM_BF16 && storage != SSAM_F16)
         return SSAM_E_INVALID;
-    }
+    const bool two_byte = storage == SSAM_BF16 || storage == SSAM_F16;
     if (nx < 1 || ny < 1 || nz < 1 || steps < 1) return SSAM_E_INVALID;
     if (nx > (1 << 30) || ny > (1 << 30) || nz > (1 << 30)) return SSAM_E_UNSUPPORTED;
     if (nx * ny > (1LL << 40) / nz) return SSAM_E_UNSUPPORTED;
@@ -569,15 +573,11 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
     } else {
         return SSAM_E_INVALID;
     }
-    const int64_t bytes = nx * ny * nz * (storage == SSAM_F64 ? 8 : 4);
+    const int64_t bytes = nx * ny * nz * elem_size(storage);
     if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
-    if (steps > 1) {
-        if (!work) return SSAM_E_INVALID;
-        if (overlaps(work, bytes, dst, bytes) || overlaps(work, bytes, src, bytes))
-            return SSAM_E_INVALID;
-    }
     const bool want_res = fixed ? fixed->out_dev != nullptr : residual_l2 != nullptr;
     Req3D rq;
+    rq.out_storage = storage;
     rq.nx = nx;
     rq.ny = ny;
     rq.nz = nz;
@@ -605,20 +605,43 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
         }
     }
     // launches: two sweeps each where the temporal form applies (radius 1, grid
-    // under 2 GiB, SSAM_3D_T != 1), the rest one sweep; a residual request runs
-    // the last sweep on its own (the single-sweep kernel forms the residual).
-    // Launches ping-pong so the last one writes dst.
-    const bool t2 = steps >= 2 && two_sweeps_ok(storage, rq) && env_int("SSAM_3D_T", 2) != 1;
+    // under 2 GiB also as the FP32 state of a 2-byte chain, SSAM_3D_T != 1), the
+    // rest one sweep; a residual request runs the last sweep on its own (the
+    // single-sweep kernel forms the residual).
+    Req3D probe = rq;
+    if (two_byte) probe.out_storage = SSAM_F32;
+    const bool t2 = steps >= 2 && two_sweeps_ok(two_byte ? SSAM_F32 : storage, probe) &&
+                    env_int("SSAM_3D_T", 2) != 1;
     const int tail = want_res ? 1 : 0;
     auto sweeps_of = [&](int left) { return (t2 && left - tail >= 2) ? 2 : 1; };
     int nl = 0;
     for (int left = steps; left > 0; left -= sweeps_of(left)) ++nl;
+    // F32 / F64: launches ping-pong through work so the last one writes dst;
+    // BF16 / F16: the state between launches ping-pongs between two FP32 grids
+    // (multi-sweep requests always name their work area, as documented, even when
+    // one launch covers them)
+    if (steps > 1) {
+        if (!work || (two_byte && !work_fp32)) return SSAM_E_INVALID;
+        const int64_t wbytes = two_byte ? 2 * nx * ny * nz * 4 : bytes;
+        if (overlaps(work, wbytes, dst, bytes) || overlaps(work, wbytes, src, bytes))
+            return SSAM_E_INVALID;
+    }
+    const bool fp32_state = two_byte && nl > 1;
+    float* state[2] = {static_cast<float*>(work), static_cast<float*>(work) + (fp32_state ? nx * ny * nz : 0)};
     const void* in = src;
     int left = steps;
     for (int i = 0; i < nl; ++i) {
         const int n = sweeps_of(left);
         left -= n;
         void* out = ((nl - 1 - i) % 2 == 0) ? dst : work;
+        int in_storage = storage;
+        rq.out_storage = storage;
+        if (fp32_state) {
+            in = i == 0 ? src : state[(i - 1) % 2];
+            out = i == nl - 1 ? dst : state[i % 2];
+            in_storage = i == 0 ? storage : SSAM_F32;
+            rq.out_storage = i == nl - 1 ? storage : SSAM_F32;
+        }
         rq.src = in;
         rq.dst = out;
         if (want_res && i == nl - 1) {
@@ -633,7 +656,7 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
                 rq.res_out = scratch;
             }
         }
-        const int rc = n == 2 ? dispatch3d_two_sweeps(storage, rq, st) : dispatch3d(storage, rq, st);
+        const int rc = n == 2 ? dispatch3d_two_sweeps(in_storage, rq, st) : dispatch3d(in_storage, rq, st);
         if (rc != SSAM_OK) {
             // launches already queued may use the scratch: let them finish first
             if (want_res && !fixed) (void)hipStreamSynchronize(st);
@@ -654,7 +677,7 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
 template <typename T>
 int ref3d(int dtype, int shape, int radius, const double* taps, const T* src, T* dst, int n,
           int iters, T* work, void* stream) {
-    return stencil3d_run(dtype, shape, radius, taps, iters, src, dst, n, n, n, work, nullptr, 0.0,
+    return stencil3d_run(dtype, shape, radius, taps, iters, src, dst, n, n, n, work, false, nullptr, 0.0,
                          static_cast<hipStream_t>(stream));
 }
 
@@ -666,8 +689,10 @@ int ssam_stencil3d(const ssam_stencil3d_desc* d, const void* src, void* dst, int
                    int64_t ny, int64_t nz, void* work, void* stream) {
     if (!d) return SSAM_E_INVALID;
     if (d->boundary != SSAM_BOUNDARY_DIRICHLET0) return SSAM_E_UNSUPPORTED;
+    if (d->flags & ~SSAM_FLAG_WORK_FP32) return SSAM_E_INVALID;
     return stencil3d_run(d->storage, d->shape, d->radius, d->coeffs, d->steps, src, dst, nx, ny,
-                         nz, work, nullptr, 0.0, static_cast<hipStream_t>(stream));
+                         nz, work, (d->flags & SSAM_FLAG_WORK_FP32) != 0, nullptr, 0.0,
+                         static_cast<hipStream_t>(stream));
 }
 
 int ssam_j3d7pt_f32(const float* src, float* dst, int n, int iters, float* work, void* stream) {
@@ -696,8 +721,9 @@ int ssam_poisson_f64(const double* src, double* dst, int n, int iters, double* w
 int ssam_jacobi19(int storage, const void* src, void* dst, int64_t nx, int64_t ny, int64_t nz,
                   int steps, void* work, double* residual_l2, void* stream) {
     // r = (2F + E - 24u)/6 = 4 * (u' - u) with u' = (2F + E)/24
+    // (BF16 / F16 storage: `work` holds two FP32 grids, the running state)
     return stencil3d_run(storage, SSAM_SHAPE_FACE_EDGE, 1, kJacobi19, steps, src, dst, nx, ny, nz,
-                         work, residual_l2, 4.0, static_cast<hipStream_t>(stream));
+                         work, true, residual_l2, 4.0, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"
@@ -737,11 +763,12 @@ extern "C" int ssam_jacobi19_noalloc(int storage, const void* src, void* dst, in
                                      int64_t ny, int64_t nz, int steps, void* work,
                                      double* residual_l2, void* stream) {
     ResidualArea area{nullptr, kNoAllocPartials, residual_l2};
-    const bool valid = src && dst && (storage == SSAM_F32 || storage == SSAM_F64) && nx >= 1 &&
+    const bool valid = src && dst && elem_size(storage) != 0 && nx >= 1 &&
                        ny >= 1 && nz >= 1 && nx <= (1 << 30) && ny <= (1 << 30) && nz <= (1 << 30);
     if (residual_l2 && valid) {
         // grids whose partials outgrow the array are rejected before any device work
         Req3D rq{};
+        rq.out_storage = storage;
         rq.nx = nx;
         rq.ny = ny;
         rq.nz = nz;
@@ -753,7 +780,7 @@ extern "C" int ssam_jacobi19_noalloc(int storage, const void* src, void* dst, in
         if (rc != SSAM_OK) return rc;
     }
     return stencil3d_run(storage, SSAM_SHAPE_FACE_EDGE, 1, kJacobi19, steps, src, dst, nx, ny, nz,
-                         work, nullptr, 4.0, static_cast<hipStream_t>(stream), &area);
+                         work, true, nullptr, 4.0, static_cast<hipStream_t>(stream), &area);
 }
 
 // ------------------------------------------------------------------ autotuner

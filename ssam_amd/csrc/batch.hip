@@ -89,8 +89,15 @@ namespace {
 
 using Plan = ssam_batch_plan_s;
 
+bool two_byte(int storage) { return storage == SSAM_BF16 || storage == SSAM_F16; }
+
+// Tile shape of a 3D group: the plan's, except that BF16 / F16 groups always run
+// the two-sweep shape (the only one instantiated for 2-byte storage and for the
+// FP32 state between their passes).
+int var3d_of(const Plan* P, const Plan::Group& g) { return two_byte(g.storage) ? kVarT2 : P->var3d; }
+
 // 3D launch form in effect: the two-sweep shape always runs as an ordinary grid.
-bool grid3d_on(const Plan* P) { return P->grid3d || kVar3D[P->var3d].t == 2; }
+bool grid3d_on(const Plan* P, const Plan::Group& g) { return P->grid3d || kVar3D[var3d_of(P, g)].t == 2; }
 
 int elem_size_b(int dtype) {
     switch (dtype) {
@@ -344,9 +351,10 @@ int build_group2d(Plan* P, Plan::Group& g) {
 // (Re)build the device tables of a 3D group with the plan's tile shape.
 template <typename S>
 int build_group3d(Plan* P, Plan::Group& g) {
-    const int variant = P->var3d, zseg = P->zseg3d;
+    using A = typename Storage<S>::acc;
+    const int variant = var3d_of(P, g), zseg = P->zseg3d;
     std::vector<BatchItem3D> h(g.ids.size());
-    std::vector<S> taps;
+    std::vector<A> taps;
     std::vector<ItemTiles> per(g.ids.size());
     for (size_t k = 0; k < g.ids.size(); ++k) {
         const ssam_batch_item& it = P->items[g.ids[k]];
@@ -355,13 +363,14 @@ int build_group3d(Plan* P, Plan::Group& g) {
         b.dst = it.dst;
         b.work = it.work;
         b.taps_off = static_cast<int>(taps.size());
-        b.var = variant == kVarPair3D ? pair_choice(it.nx, sizeof(S)) : variant;
+        b.var = variant == kVarPair3D ? pair_choice(it.nx, sizeof(A)) : variant;
         b.steps = passes3d(static_cast<int>(it.steps), variant);
+        b.fp32_state = two_byte(g.storage) && b.steps > 1;
         const int T = kVar3D[variant].t;
-        const int V = var3d_v(b.var, sizeof(S)), NY = kVar3D[b.var].ny;
+        const int V = var3d_v(b.var, sizeof(A)), NY = kVar3D[b.var].ny;
         const int OW = (kWave - 2 * (T == 2 && V == 1 ? 2 : 1)) * V;  // halo lanes: two columns per sweep
         const int nt = g.kind ? 19 : 7;
-        for (int t = 0; t < nt; ++t) taps.push_back(static_cast<S>(it.coeffs[t]));
+        for (int t = 0; t < nt; ++t) taps.push_back(static_cast<A>(it.coeffs[t]));
         Grid3D& gr = b.g;
         gr.nx = static_cast<int>(it.nx);
         gr.ny = static_cast<int>(it.ny);
@@ -371,9 +380,10 @@ int build_group3d(Plan* P, Plan::Group& g) {
         b.nstrips = static_cast<int>((it.nx + OW - 1) / OW);
         b.nyb = static_cast<int>((it.ny + NY - 1) / NY);
         const uintptr_t al = static_cast<uintptr_t>(V) * sizeof(S);
+        const uintptr_t alw = static_cast<uintptr_t>(V) * (b.fp32_state ? sizeof(float) : sizeof(S));
         b.vec = (it.nx % V == 0) && (reinterpret_cast<uintptr_t>(it.src) % al == 0) &&
                 (reinterpret_cast<uintptr_t>(it.dst) % al == 0) &&
-                (!it.work || reinterpret_cast<uintptr_t>(it.work) % al == 0);
+                (!it.work || reinterpret_cast<uintptr_t>(it.work) % alw == 0);
         const int zs_len = std::min<int>(zseg, gr.nz);
         for (int z0 = 0; z0 < gr.nz; z0 += zs_len) {
             const int planes = std::min(zs_len, gr.nz - z0);
@@ -388,14 +398,15 @@ int build_group3d(Plan* P, Plan::Group& g) {
         }
     }
     std::vector<int4> tiles;
-    if (grid3d_on(P)) pack_single(per, &tiles, &g.qs);
+    if (grid3d_on(P, g)) pack_single(per, &tiles, &g.qs);
     else pack_queues(per, &tiles, &g.qs);
     g.qs.affinity = P->aff3d;
     g.qs.alternate = P->alt3d;
     free_group(g);
     g.ntiles = static_cast<int>(tiles.size());
-    g.blocks = grid3d_on(P) ? (g.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock
-                         : resident_blocks(batch3d_kernel_fn(g.storage, variant, g.kind, false), P->wps3d);
+    g.blocks = grid3d_on(P, g)
+                   ? (g.ntiles + kBatchWavesPerBlock - 1) / kBatchWavesPerBlock
+                   : resident_blocks(batch3d_kernel_fn(g.storage, g.storage, variant, g.kind, false), P->wps3d);
     int rc = upload(&g.d_items, h);
     if (rc == SSAM_OK) rc = upload(&g.d_tiles, tiles);
     if (rc == SSAM_OK) rc = upload(&g.d_taps, taps);
@@ -411,7 +422,12 @@ int build_group(Plan* P, Plan::Group& g) {
             default: return build_group2d<f16_t>(P, g);
         }
     }
-    return g.storage == SSAM_F64 ? build_group3d<double>(P, g) : build_group3d<float>(P, g);
+    switch (g.storage) {
+        case SSAM_F64: return build_group3d<double>(P, g);
+        case SSAM_F32: return build_group3d<float>(P, g);
+        case SSAM_BF16: return build_group3d<bf16_t>(P, g);
+        default: return build_group3d<f16_t>(P, g);
+    }
 }
 
 int build_all(Plan* P) {
@@ -493,6 +509,7 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
                 d.radius = it.radius;
                 d.steps = it.steps;
                 d.boundary = SSAM_BOUNDARY_DIRICHLET0;
+                if (two_byte(it.storage)) d.flags = SSAM_FLAG_WORK_FP32;  // work: two FP32 grids
                 d.coeffs = it.coeffs;
                 rc = ssam_stencil3d(&d, it.src, it.dst, it.nx, it.ny, it.nz, it.work, st);
             }
@@ -501,7 +518,6 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
     }
     // passes: one sweep each, or two (temporal variants; the last pass of an odd
     // sweep count runs one)
-    const bool t2 = kVar3D[P->var3d].t == 2;
     int maxsteps = 0;
     for (auto& g : P->groups)
         if (g.ndim == 3) maxsteps = std::max(maxsteps, g.steps);
@@ -509,9 +525,18 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
         for (auto& g : P->groups) {
             if (g.ndim != 3) continue;
             int* cnt = counters + kCounterSet * ctr++;
-            const int npass = passes3d(g.steps, P->var3d);
+            const int var = var3d_of(P, g);
+            const bool t2 = kVar3D[var].t == 2;
+            const int npass = passes3d(g.steps, var);
             if (!(parts & kRun3D) || s >= npass) continue;
             int single = t2 && (g.steps % 2 == 1) && s == npass - 1;
+            // storage read and written by this pass (FP32 between the passes of a
+            // BF16 / F16 group)
+            int in_st = g.storage, out_st = g.storage;
+            if (two_byte(g.storage) && npass > 1) {
+                in_st = s == 0 ? g.storage : SSAM_F32;
+                out_st = s == npass - 1 ? g.storage : SSAM_F32;
+            }
             const int blocks = std::max(1, std::min(g.blocks, (g.ntiles + kBatchWavesPerBlock - 1) /
                                                                   kBatchWavesPerBlock));
             const BatchItem3D* items = static_cast<const BatchItem3D*>(g.d_items);
@@ -520,7 +545,8 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             int step = s;
             const void* taps = g.d_taps;
             void* args[] = {&items, &tiles, &qs, &cnt, &taps, &step, &single};
-            const void* fn = batch3d_kernel_fn(g.storage, P->var3d, g.kind, grid3d_on(P));
+            const void* fn = batch3d_kernel_fn(in_st, out_st, var, g.kind, grid3d_on(P, g));
+            if (!fn) return SSAM_E_UNSUPPORTED;
             if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
                     hipSuccess ||
                 hipGetLastError() != hipSuccess)
@@ -580,20 +606,19 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
             else g2[it.storage].push_back(i);
         } else if (it.ndim == 3) {
             if (it.nx < 1 || it.ny < 1 || it.nz < 1) return SSAM_E_INVALID;
-            if (it.storage != SSAM_F32 && it.storage != SSAM_F64) return SSAM_E_UNSUPPORTED;
             if (it.steps > 1 && !it.work) return SSAM_E_INVALID;
             const int64_t bytes = it.nx * it.ny * it.nz * es;
             const char* a = static_cast<const char*>(it.src);
             const char* b = static_cast<const char*>(it.dst);
             if (a < b + bytes && b < a + bytes) return SSAM_E_INVALID;
+            // (BF16 / F16: the FP32 state between passes must fit the buffer too)
             if (it.radius == 1 && (it.shape == SSAM_SHAPE_STAR || it.shape == SSAM_SHAPE_FACE_EDGE) &&
-                grid3d_fits_buffer(it.nx, it.ny, it.nz, es)) {
+                grid3d_fits_buffer(it.nx, it.ny, it.nz, std::max(es, 4))) {
                 // tile fields: first plane in 20 bits, plane count in 11
                 if (it.nz >= (1 << 20) || it.nx > (1 << 30) || it.ny > (1 << 30))
                     return SSAM_E_UNSUPPORTED;
                 const int64_t key = (static_cast<int64_t>(it.steps) * 2 +
-                                     (it.shape == SSAM_SHAPE_FACE_EDGE ? 1 : 0)) * 2 +
-                                    (it.storage == SSAM_F64 ? 1 : 0);
+                                     (it.shape == SSAM_SHAPE_FACE_EDGE ? 1 : 0)) * 4 + it.storage;
                 auto pos = std::find(g3key.begin(), g3key.end(), key);
                 if (pos == g3key.end()) {
                     g3key.push_back(key);
@@ -760,7 +785,7 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
         // the persistent grid's occupancy, queue affinity and step direction
         search({&P->var3d, &P->zseg3d, &P->grid3d},
                {{0, 1, 2, 3, kVarPair3D, kVarT2}, {16, 32, 64}, {0, 1}}, kRun3D);
-        if (!grid3d_on(P)) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
+        if (!P->grid3d && kVar3D[P->var3d].t != 2) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
         search({&P->aff3d, &P->alt3d}, {{0, 1}, {0, 1}}, kRun3D);
     }
     const int rc2 = build_all(P);
@@ -778,7 +803,7 @@ int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* 
         sched[4] = plan->mfma2d;
         sched[5] = plan->wps2d;
         sched[6] = plan->wps3d;
-        sched[7] = grid3d_on(plan) ? 1 : 0;
+        sched[7] = plan->grid3d || kVar3D[plan->var3d].t == 2 ? 1 : 0;
         sched[8] = plan->grid2d;
     }
     if (seg_rows_2d)

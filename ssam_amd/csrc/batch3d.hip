@@ -9,11 +9,11 @@ namespace ssam {
 
 template <typename S, int SHAPE, bool GRID> Batch3DFn<S> batch3d_fn_shape(int variant) {
     switch (variant) {
-        case 0: return batch3d_kernel<S, 0, SHAPE, GRID>;
-        case 1: return batch3d_kernel<S, 1, SHAPE, GRID>;
-        case 2: return batch3d_kernel<S, 2, SHAPE, GRID>;
-        case 3: return batch3d_kernel<S, 3, SHAPE, GRID>;
-        default: return batch3d_kernel<S, kVarPair3D, SHAPE, GRID>;
+        case 0: return batch3d_kernel<S, S, 0, SHAPE, GRID>;
+        case 1: return batch3d_kernel<S, S, 1, SHAPE, GRID>;
+        case 2: return batch3d_kernel<S, S, 2, SHAPE, GRID>;
+        case 3: return batch3d_kernel<S, S, 3, SHAPE, GRID>;
+        default: return batch3d_kernel<S, S, kVarPair3D, SHAPE, GRID>;
     }
 }
 
@@ -26,10 +26,13 @@ template <typename S> const void* batch3d_fn_t(int variant, int shape3d, bool gr
     return reinterpret_cast<const void*>(f);
 }
 
-const void* batch3d_kernel_fn(int storage, int variant, int shape3d, bool grid) {
-    if (kVar3D[variant].t == 2) return batch3d_t2_kernel_fn(storage, shape3d);
-    return storage == SSAM_F64 ? batch3d_fn_t<double>(variant, shape3d, grid)
-                               : batch3d_fn_t<float>(variant, shape3d, grid);
+const void* batch3d_kernel_fn(int storage, int out_storage, int variant, int shape3d, bool grid) {
+    if (kVar3D[variant].t == 2) return batch3d_t2_kernel_fn(storage, out_storage, shape3d);
+    // one-sweep shapes: F32 and F64 (BF16 / F16 items always run the two-sweep shape)
+    if (storage != out_storage) return nullptr;
+    if (storage == SSAM_F64) return batch3d_fn_t<double>(variant, shape3d, grid);
+    if (storage == SSAM_F32) return batch3d_fn_t<float>(variant, shape3d, grid);
+    return nullptr;
 }
 
 }  // namespace ssam

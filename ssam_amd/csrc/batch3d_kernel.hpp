@@ -16,24 +16,28 @@ template <typename S> struct Waves3D<S, kVarPair3D> {
     static constexpr int value = a < b ? a : b;
 };
 
-template <typename S, int SHAPE, int VAR>
-__device__ __forceinline__ void run_tile3d(const S* in, S* out, const BatchItem3D& b, const S* w, int strip,
-                                           int yb, int zs, int ze, bool single) {
-    using T = Var3DT<S, VAR>;
+template <typename SI, typename SO, int SHAPE, int VAR>
+__device__ __forceinline__ void run_tile3d(const SI* in, SO* out, const BatchItem3D& b,
+                                           const typename Storage<SI>::acc* w, int strip, int yb, int zs, int ze,
+                                           bool single) {
+    using T = Var3DT<SI, VAR>;
     if constexpr (T::T == 2) {
-        Stencil3DHL2<S, SHAPE, T::V, T::NY, T::PF>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze,
-                                                              single);
+        Stencil3DHL2<SI, SHAPE, T::V, T::NY, T::PF, SO>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze,
+                                                                   single);
     } else {
         (void)single;
-        Stencil3DHL<S, SHAPE, T::V, T::NY, T::PF>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze);
+        Stencil3DHL<SI, SHAPE, T::V, T::NY, T::PF, SO>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze);
     }
 }
 
 // VAR == kVarPair3D: each item runs the tile shape the plan chose for it.
-template <typename S, int VAR, int SHAPE, bool GRID>
-__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(Waves3D<S, VAR>::value)))
+// SI / SO: storage read and written by this pass (a BF16 / F16 item of several
+// passes reads its storage type first, writes it last, and keeps FP32 between).
+template <typename SI, typename SO, int VAR, int SHAPE, bool GRID>
+__global__ __launch_bounds__(kWave * kBatchWavesPerBlock) __attribute__((amdgpu_waves_per_eu(Waves3D<SI, VAR>::value)))
 void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restrict__ tiles, QueueSet qs,
-                    int* __restrict__ counters, const S* __restrict__ taps, int step, int single) {
+                    int* __restrict__ counters, const typename Storage<SI>::acc* __restrict__ taps, int step,
+                    int single) {
     auto body = [&](int t) {
         const int4 e = tiles[t];
         const int it = __builtin_amdgcn_readfirstlane(e.x);
@@ -43,17 +47,27 @@ void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restric
         const int zs = zw & 0xFFFFF, ze = zs + (zw >> 20);
         const BatchItem3D& b = items[it];
         // passes ping-pong so the last pass writes dst (same rule as ssam_stencil3d
-        // with passes for sweeps); b.steps holds the item's pass count
-        const void* in = step == 0 ? b.src : (((b.steps - step) % 2 == 0) ? b.dst : b.work);
-        void* out = ((b.steps - 1 - step) % 2 == 0) ? b.dst : b.work;
-        const S* src = static_cast<const S*>(in);
-        S* dst = static_cast<S*>(out);
-        const S* w = taps + b.taps_off;
-        if constexpr (VAR == kVarPair3D) {
-            if (b.var == kPairB) run_tile3d<S, SHAPE, kPairB>(src, dst, b, w, strip, yb, zs, ze, false);
-            else run_tile3d<S, SHAPE, kPairA>(src, dst, b, w, strip, yb, zs, ze, false);
+        // with passes for sweeps); b.steps holds the item's pass count.  An FP32
+        // state alternates between the two FP32 grids of work.
+        const void* in;
+        void* out;
+        if (b.fp32_state) {
+            const size_t n = static_cast<size_t>(b.g.nx) * b.g.ny * b.g.nz;
+            float* st0 = static_cast<float*>(b.work);
+            in = step == 0 ? b.src : static_cast<const void*>(st0 + ((step - 1) % 2) * n);
+            out = step == b.steps - 1 ? b.dst : static_cast<void*>(st0 + (step % 2) * n);
         } else {
-            run_tile3d<S, SHAPE, VAR>(src, dst, b, w, strip, yb, zs, ze, single != 0);
+            in = step == 0 ? b.src : (((b.steps - step) % 2 == 0) ? b.dst : b.work);
+            out = ((b.steps - 1 - step) % 2 == 0) ? b.dst : b.work;
+        }
+        const SI* src = static_cast<const SI*>(in);
+        SO* dst = static_cast<SO*>(out);
+        const typename Storage<SI>::acc* w = taps + b.taps_off;
+        if constexpr (VAR == kVarPair3D) {
+            if (b.var == kPairB) run_tile3d<SI, SO, SHAPE, kPairB>(src, dst, b, w, strip, yb, zs, ze, false);
+            else run_tile3d<SI, SO, SHAPE, kPairA>(src, dst, b, w, strip, yb, zs, ze, false);
+        } else {
+            run_tile3d<SI, SO, SHAPE, VAR>(src, dst, b, w, strip, yb, zs, ze, single != 0);
         }
     };
     if constexpr (GRID) {
@@ -67,6 +81,14 @@ void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restric
 }
 
 template <typename S>
-using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, const S*, int, int);
+using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, const typename Storage<S>::acc*, int,
+                           int);
+
+// The two-sweep kernel of one storage pair and shape (batch3d_t2*.hip).
+template <typename SI, typename SO>
+const void* batch3d_t2_fn(int shape3d) {
+    return shape3d ? reinterpret_cast<const void*>(batch3d_kernel<SI, SO, kVarT2, k3FaceEdge, true>)
+                   : reinterpret_cast<const void*>(batch3d_kernel<SI, SO, kVarT2, k3Star, true>);
+}
 
 }  // namespace ssam

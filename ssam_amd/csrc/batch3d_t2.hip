@@ -4,12 +4,12 @@
 
 namespace ssam {
 
-const void* batch3d_t2_kernel_fn(int storage, int shape3d) {
-    if (storage == SSAM_F64)
-        return shape3d ? reinterpret_cast<const void*>(batch3d_kernel<double, kVarT2, k3FaceEdge, true>)
-                       : reinterpret_cast<const void*>(batch3d_kernel<double, kVarT2, k3Star, true>);
-    return shape3d ? reinterpret_cast<const void*>(batch3d_kernel<float, kVarT2, k3FaceEdge, true>)
-                   : reinterpret_cast<const void*>(batch3d_kernel<float, kVarT2, k3Star, true>);
+const void* batch3d_t2_kernel_fn(int storage, int out_storage, int shape3d) {
+    if (storage == SSAM_BF16 || out_storage == SSAM_BF16) return batch3d_t2_kernel_fn_bf16(storage, out_storage, shape3d);
+    if (storage == SSAM_F16 || out_storage == SSAM_F16) return batch3d_t2_kernel_fn_f16(storage, out_storage, shape3d);
+    if (storage != out_storage) return nullptr;
+    if (storage == SSAM_F64) return batch3d_t2_fn<double, double>(shape3d);
+    return batch3d_t2_fn<float, float>(shape3d);
 }
 
 }  // namespace ssam

@@ -61,6 +61,8 @@ struct BatchItem3D {
     int steps;     // passes over the grid (sweeps / sweeps per pass, rounded up)
     int taps_off;
     int var;       // tile shape of this item's tiles (a kVar3D index)
+    int fp32_state;  // BF16 / F16 item of several passes: the state between passes
+                     // ping-pongs between two FP32 grids in `work`
 };
 
 constexpr int kBatchWavesPerBlock = 4;
@@ -168,11 +170,13 @@ constexpr int kDefaultVar3D = kVarT2;
 constexpr int kDefaultZseg3D = 16;
 constexpr int kDefaultSeg2D = 64;
 
+// (by the accumulation type: a BF16 / F16 chain and its FP32 state share a shape)
 template <typename S, int VAR> struct Var3DT {
+    using A = typename Storage<S>::acc;
     // FP64 runs one column per lane (a plane row of NY+2 doubles per lane)
-    static constexpr int V = sizeof(S) == 8 ? 1 : kVar3D[VAR].v;
+    static constexpr int V = sizeof(A) == 8 ? 1 : kVar3D[VAR].v;
     static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
-    static constexpr int WPE = sizeof(S) == 8 ? (NY >= 16 || kVar3D[VAR].t == 2 ? 1 : 2) : kVar3D[VAR].wpe;
+    static constexpr int WPE = sizeof(A) == 8 ? (NY >= 16 || kVar3D[VAR].t == 2 ? 1 : 2) : kVar3D[VAR].wpe;
     static constexpr int T = kVar3D[VAR].t;
 };
 
@@ -204,7 +208,11 @@ inline int pair_choice(long long nx, int elem_bytes) {
 // (split form) or -1 (fused); shape3d: 0 7-point star, 1 19-point face+edge;
 // grid: one tile per wave of an ordinary grid instead of a persistent grid.
 const void* batch2d_kernel_fn(int storage, int kind, bool grid);
-const void* batch3d_kernel_fn(int storage, int variant, int shape3d, bool grid);
-const void* batch3d_t2_kernel_fn(int storage, int shape3d);
+// 3D passes read `storage` and write `out_storage` (the same, or a 2-byte type to
+// / from its FP32 state; those pairs exist for the two-sweep shape only).
+const void* batch3d_kernel_fn(int storage, int out_storage, int variant, int shape3d, bool grid);
+const void* batch3d_t2_kernel_fn(int storage, int out_storage, int shape3d);
+const void* batch3d_t2_kernel_fn_bf16(int storage, int out_storage, int shape3d);
+const void* batch3d_t2_kernel_fn_f16(int storage, int out_storage, int shape3d);
 
 }  // namespace ssam

@@ -53,6 +53,12 @@ template <> struct Storage<f16_t> {
         return static_cast<float>(h);
     }
     __device__ static inline f16_t store(float v) {
+        // the FP32 result is rounded to FP16 as a value of its own: without the
+        // opaque copy the compiler folds the last FMA and the conversion into
+        // v_fma_mixlo_f16 (one rounding of a*b+c straight to FP16), which can
+        // differ from rounding the FP32 sum in a tie, so the same sweeps would
+        // round differently in a fused launch and in a chain of launches
+        asm("" : "+v"(v));
         _Float16 h = static_cast<_Float16>(v);
         f16_t r;
         __builtin_memcpy(&r.bits, &h, 2);
@@ -150,6 +156,74 @@ __device__ __forceinline__ void buf_store(BufRsrc r, uint32_t off, const T (&in)
     if constexpr (W == 1) __builtin_amdgcn_raw_buffer_store_b32(v, r, static_cast<int>(off), 0, 0);
     else if constexpr (W == 2) __builtin_amdgcn_raw_buffer_store_b64(v, r, static_cast<int>(off), 0, 0);
     else __builtin_amdgcn_raw_buffer_store_b128(v, r, static_cast<int>(off), 0, 0);
+}
+
+// V consecutive elements of storage type S at byte offset `off` of a raw buffer,
+// in the accumulation type.  VEC: one access for the V elements (their bytes a
+// multiple of 4); otherwise an access per element (16-bit ones for BF16 / F16).
+template <typename S, int V, bool VEC>
+__device__ __forceinline__ void buf_load_elems(BufRsrc r, uint32_t off, typename Storage<S>::acc (&out)[V]) {
+    using A = typename Storage<S>::acc;
+    constexpr uint32_t ES = sizeof(S);
+    if constexpr (sizeof(S) == sizeof(A)) {
+        if constexpr (VEC) {
+            buf_load<A, V>(r, off, out);
+        } else {
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                A e[1];
+                buf_load<A, 1>(r, off + static_cast<uint32_t>(j) * ES, e);
+                out[j] = e[0];
+            }
+        }
+    } else if constexpr (VEC && (V * sizeof(S)) % 4 == 0) {
+        uint32_t w[V / 2];
+        buf_load<uint32_t, V / 2>(r, off, w);
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            S s;
+            s.bits = static_cast<uint16_t>((j & 1) ? (w[j / 2] >> 16) : (w[j / 2] & 0xffffu));
+            out[j] = Storage<S>::load(s);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            S s;
+            s.bits = __builtin_amdgcn_raw_buffer_load_b16(r, static_cast<int>(off + static_cast<uint32_t>(j) * ES), 0, 0);
+            out[j] = Storage<S>::load(s);
+        }
+    }
+}
+
+// Stores V values (accumulation type) as storage type S: VEC, one access at
+// so[0] + roff; otherwise element j at so[j] + roff (an offset at or past the
+// buffer end drops that element).
+template <typename S, int V, bool VEC>
+__device__ __forceinline__ void buf_store_elems(BufRsrc r, const uint32_t (&so)[V], uint32_t roff,
+                                                const typename Storage<S>::acc (&v)[V]) {
+    using A = typename Storage<S>::acc;
+    if constexpr (sizeof(S) == sizeof(A)) {
+        if constexpr (VEC) {
+            buf_store<A, V>(r, so[0] + roff, v);
+        } else {
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const A e[1] = {v[j]};
+                buf_store<A, 1>(r, so[j] + roff, e);
+            }
+        }
+    } else if constexpr (VEC && (V * sizeof(S)) % 4 == 0) {
+        uint32_t w[V / 2];
+#pragma unroll
+        for (int j = 0; j < V; j += 2)
+            w[j / 2] = static_cast<uint32_t>(Storage<S>::store(v[j]).bits) |
+                       (static_cast<uint32_t>(Storage<S>::store(v[j + 1]).bits) << 16);
+        buf_store<uint32_t, V / 2>(r, so[0] + roff, w);
+    } else {
+#pragma unroll
+        for (int j = 0; j < V; ++j)
+            __builtin_amdgcn_raw_buffer_store_b16(Storage<S>::store(v[j]).bits, r, static_cast<int>(so[j] + roff), 0, 0);
+    }
 }
 
 // ---------------------------------------------------------------- strip I/O

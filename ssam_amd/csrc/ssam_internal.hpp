@@ -53,6 +53,8 @@ int device_tap_copy(const void* host, size_t bytes, const void** dev);
 struct Req3D {
     const void* src;
     void* dst;
+    int out_storage;     // ssam_dtype of dst (the request's storage, or FP32 / 2-byte
+                         // at the ends of a chain with FP32 running state)
     int64_t nx, ny, nz;
     int shape;           // Shape3D: 0 star, 1 face+edge (19 points)
     int radius;
@@ -65,6 +67,7 @@ struct Req3D {
     double res_scale;
 };
 
+// One sweep from `storage` (src) to rq.out_storage (dst).
 int dispatch3d(int storage, const Req3D& rq, hipStream_t st);
 // Two sweeps in one launch (radius 1, grid under 2 GiB); SSAM_E_UNSUPPORTED else.
 bool two_sweeps_ok(int storage, const Req3D& rq);

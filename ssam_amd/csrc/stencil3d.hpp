@@ -19,6 +19,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ssam_device.hpp"
 
 namespace ssam {
@@ -61,9 +63,12 @@ struct Params3D {
 
 constexpr int kWaves3D = 4;
 
-template <typename S, int SHAPE, int R, int V, int NY, int PF>
+// S: storage of src; SO: storage of dst (default S) -- a chain of sweeps over
+// BF16 / FP16 grids keeps its running state in FP32 between launches.
+template <typename S, int SHAPE, int R, int V, int NY, int PF, typename SO = S>
 struct Stencil3D {
     using A = typename Storage<S>::acc;
+    static_assert(std::is_same<A, typename Storage<SO>::acc>::value, "one accumulation type");
     static constexpr int NT = ntaps3(SHAPE, R);
     static constexpr int NROW = NY + 2 * R;   // rows cached per plane
     static constexpr int NP = 2 * R + 1 + PF; // planes in the ring
@@ -75,10 +80,10 @@ struct Stencil3D {
 
     struct Ctx {
         const S* src;
-        S* dst;
+        SO* dst;
         Params3D p;
         int lane, x0, y0;
-        uint32_t xoff;
+        uint32_t xoff, xoff_out;  // byte offsets of the lane's columns in a src / dst row
         int hx;  // first column of this lane's halo load (always in [0, nx))
         bool full;
     };
@@ -161,15 +166,15 @@ struct Stencil3D {
             if (q < c.p.nz && y < c.p.ny && zq0 + KT >= 0) {
                 char* row = reinterpret_cast<char*>(c.dst) +
                             (static_cast<long long>(q) * c.p.sz + static_cast<long long>(y) * c.p.sy) *
-                                static_cast<long long>(sizeof(S));
+                                static_cast<long long>(sizeof(SO));
                 if (!CHK || (c.full && c.p.vec_ok)) {
-                    store_strip_vec<S, V>(reinterpret_cast<S*>(row + c.xoff), out);
+                    store_strip_vec<SO, V>(reinterpret_cast<SO*>(row + c.xoff_out), out);
                 } else {
 #pragma unroll
                     for (int j = 0; j < V; ++j)
                         if (c.x0 + j < c.p.nx && c.x0 + j >= 0)
-                            gst<S>(row + c.xoff + static_cast<uint32_t>(j * sizeof(S)),
-                                Storage<S>::store(out[j]));
+                            gst<SO>(row + c.xoff_out + static_cast<uint32_t>(j * sizeof(SO)),
+                                    Storage<SO>::store(out[j]));
                 }
             }
         }
@@ -198,7 +203,7 @@ struct Stencil3D {
 
     // One tile: x-strip `strip`, y-block `yb`, output planes [zs, ze) of the grid
     // described by p (all wave-uniform).  Returns this lane's residual^2 sum.
-    __device__ static __forceinline__ double run_tile(const S* __restrict__ src, S* __restrict__ dst,
+    __device__ static __forceinline__ double run_tile(const S* __restrict__ src, SO* __restrict__ dst,
                                                       const Params3D& p, const A* __restrict__ w,
                                                       int strip, int yb, int zs, int ze) {
         const int lane = threadIdx.x & (kWave - 1);
@@ -211,6 +216,7 @@ struct Stencil3D {
         c.y0 = yb * NY;
         c.full = c.x0 + V <= p.nx;
         c.xoff = static_cast<uint32_t>(c.x0) * static_cast<uint32_t>(sizeof(S));
+        c.xoff_out = static_cast<uint32_t>(c.x0) * static_cast<uint32_t>(sizeof(SO));
         {
             int hx = c.x0;  // other lanes: any in-range column (their halo is unused)
             if (lane == 0) hx = c.x0 - R;
@@ -222,7 +228,7 @@ struct Stencil3D {
         return interior ? stream<false>(c, w, zs, ze) : stream<true>(c, w, zs, ze);
     }
 
-    __device__ static void run(const S* __restrict__ src, S* __restrict__ dst, const Params3D& p,
+    __device__ static void run(const S* __restrict__ src, SO* __restrict__ dst, const Params3D& p,
                                const A* __restrict__ w) {
         const int lane = threadIdx.x & (kWave - 1);
         const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -248,14 +254,14 @@ struct Stencil3D {
     }
 };
 
-template <typename S, int SHAPE, int R, int V, int NY, int PF>
+template <typename S, int SHAPE, int R, int V, int NY, int PF, typename SO = S>
 __global__ __launch_bounds__(kWave * kWaves3D) void ssam3d_kernel(
     Taps3<typename Storage<S>::acc, ntaps3(SHAPE, R)> tp, const S* __restrict__ src,
-    S* __restrict__ dst, Params3D p) {
+    SO* __restrict__ dst, Params3D p) {
     (void)tp;
     using A = typename Storage<S>::acc;
     const A* __restrict__ w = (const A*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Stencil3D<S, SHAPE, R, V, NY, PF>::run(src, dst, p, w);
+    Stencil3D<S, SHAPE, R, V, NY, PF, SO>::run(src, dst, p, w);
 }
 
 }  // namespace ssam

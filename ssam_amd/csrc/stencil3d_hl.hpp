@@ -21,6 +21,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ssam_device.hpp"
 #include "stencil3d.hpp"
 
@@ -38,26 +40,30 @@ inline bool grid3d_fits_buffer(long long nx, long long ny, long long nz, int ele
     return nx * ny * nz * elem_bytes < (1LL << 31) - 4096;
 }
 
-template <typename S, int SHAPE, int V, int NY, int PF>
+// SI: storage of src; SO: storage of dst (default SI).  A chain of launches over
+// BF16 / FP16 grids keeps its running state in FP32: S -> FP32, FP32 -> FP32 and
+// FP32 -> S passes.  Accumulation in Storage<SI>::acc (FP32, or FP64 for FP64).
+template <typename SI, int SHAPE, int V, int NY, int PF, typename SO = SI>
 struct Stencil3DHL {
-    using A = typename Storage<S>::acc;
-    static_assert(sizeof(S) == sizeof(A), "F32 / F64 storage");
+    using A = typename Storage<SI>::acc;
+    static_assert(std::is_same<A, typename Storage<SO>::acc>::value, "one accumulation type");
     static constexpr int R = 1;
     static constexpr int NT = ntaps3(SHAPE, R);
     static constexpr int NROW = NY + 2;     // rows cached per plane
     static constexpr int NP = 3 + PF;       // planes in the ring
     static constexpr int OW = (kWave - 2) * V;
-    static constexpr uint32_t ES = sizeof(S);
+    static constexpr uint32_t ESI = sizeof(SI), ESO = sizeof(SO);
 
     struct Plane { A v[NROW][V]; };
 
     struct Ctx {
         BufRsrc src, dst;
         int ny, nz;
-        uint32_t sy, sz;      // byte strides
+        uint32_t syi, szi;    // byte strides of src
+        uint32_t syo, szo;    // ... of dst
         int y0;
-        uint32_t lo;          // byte offset of the lane's first column in its row
-        uint32_t so[V];       // store offset of each column (kBufOOB: not stored)
+        uint32_t lo;          // byte offset of the lane's first column in its src row
+        uint32_t so[V];       // dst offset of each column in its row (kBufOOB: not stored)
         bool xin[V];          // interior column
     };
 
@@ -65,21 +71,10 @@ struct Stencil3DHL {
     // otherwise V single-element accesses.
     template <bool VEC>
     __device__ static __forceinline__ void load_plane(const Ctx& c, int z, Plane& P) {
-        const uint32_t pz = static_cast<uint32_t>(z) * c.sz + static_cast<uint32_t>(c.y0 - 1) * c.sy;
+        const uint32_t pz = static_cast<uint32_t>(z) * c.szi + static_cast<uint32_t>(c.y0 - 1) * c.syi;
 #pragma unroll
-        for (int i = 0; i < NROW; ++i) {
-            const uint32_t off = c.lo + pz + static_cast<uint32_t>(i) * c.sy;
-            if constexpr (VEC) {
-                buf_load<A, V>(c.src, off, P.v[i]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < V; ++j) {
-                    A e[1];
-                    buf_load<A, 1>(c.src, off + static_cast<uint32_t>(j) * ES, e);
-                    P.v[i][j] = e[0];
-                }
-            }
-        }
+        for (int i = 0; i < NROW; ++i)
+            buf_load_elems<SI, V, VEC>(c.src, c.lo + pz + static_cast<uint32_t>(i) * c.syi, P.v[i]);
     }
 
     __device__ static __forceinline__ A at(const Plane& P, int i, int j, int dx) {
@@ -98,7 +93,7 @@ struct Stencil3DHL {
         asm volatile("" : "+s"(zero));
         const A* __restrict__ tw = w + zero;
         const bool zin = q >= 1 && q < c.nz - 1;
-        const uint32_t qz = static_cast<uint32_t>(q) * c.sz;
+        const uint32_t qz = static_cast<uint32_t>(q) * c.szo;
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
             const int y = c.y0 + i;
@@ -115,18 +110,7 @@ struct Stencil3DHL {
                 }
                 out[j] = (yin && c.xin[j]) ? acc : A(0);
             }
-            if (y < c.ny) {
-                const uint32_t roff = qz + static_cast<uint32_t>(y) * c.sy;
-                if constexpr (VEC) {
-                    buf_store<A, V>(c.dst, c.so[0] + roff, out);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < V; ++j) {
-                        const A e[1] = {out[j]};
-                        buf_store<A, 1>(c.dst, c.so[j] + roff, e);
-                    }
-                }
-            }
+            if (y < c.ny) buf_store_elems<SO, V, VEC>(c.dst, c.so, qz + static_cast<uint32_t>(y) * c.syo, out);
         }
     }
 
@@ -168,27 +152,28 @@ struct Stencil3DHL {
     // One tile: strip `strip`, y-block `yb`, output planes [zs, ze) (wave-uniform)
     // of a dense grid (sy = nx, sz = nx*ny) that fits grid3d_fits_buffer.
     // vec: nx % V == 0 and src/dst aligned to V elements.
-    __device__ static __forceinline__ void run_tile(const S* __restrict__ src, S* __restrict__ dst,
+    __device__ static __forceinline__ void run_tile(const SI* __restrict__ src, SO* __restrict__ dst,
                                                     const Grid3D& g, bool vec, const A* __restrict__ w,
                                                     int strip, int yb, int zs, int ze) {
         const int lane = threadIdx.x & (kWave - 1);
-        const uint32_t bytes = static_cast<uint32_t>(g.nx) * static_cast<uint32_t>(g.ny) *
-                               static_cast<uint32_t>(g.nz) * ES;
+        const uint32_t n = static_cast<uint32_t>(g.nx) * static_cast<uint32_t>(g.ny) * static_cast<uint32_t>(g.nz);
         Ctx c;
-        c.src = buf_rsrc(src, bytes);
-        c.dst = buf_rsrc(dst, bytes);
+        c.src = buf_rsrc(src, n * ESI);
+        c.dst = buf_rsrc(dst, n * ESO);
         c.ny = g.ny;
         c.nz = g.nz;
-        c.sy = static_cast<uint32_t>(g.sy) * ES;
-        c.sz = static_cast<uint32_t>(g.sz) * ES;
+        c.syi = static_cast<uint32_t>(g.sy) * ESI;
+        c.szi = static_cast<uint32_t>(g.sz) * ESI;
+        c.syo = static_cast<uint32_t>(g.sy) * ESO;
+        c.szo = static_cast<uint32_t>(g.sz) * ESO;
         c.y0 = yb * NY;
         const int x0 = strip * OW + (lane - 1) * V;
-        c.lo = static_cast<uint32_t>(x0) * ES;  // x0 = -V wraps: reads past the end
+        c.lo = static_cast<uint32_t>(x0) * ESI;  // x0 = -V wraps: reads past the end
         const bool store_lane = lane >= 1 && lane < kWave - 1;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
             const int x = x0 + j;
-            c.so[j] = (store_lane && x < g.nx) ? c.lo + static_cast<uint32_t>(j) * ES : kBufOOB;
+            c.so[j] = (store_lane && x < g.nx) ? static_cast<uint32_t>(x) * ESO : kBufOOB;
             c.xin[j] = x >= 1 && x < g.nx - 1;
         }
         if (V == 1 || vec) stream<true>(c, w, zs, ze);
@@ -207,10 +192,12 @@ constexpr int clcm(int a, int b) { return a / cgcd3(a, b) * b; }
 // grid's interior are 0, exactly what the first sweep's output holds there
 // (zero Dirichlet), so two passes of this body equal four single sweeps bit for
 // bit.  Columns: 2*HL halo lanes (one lane of V >= 2 columns, two of V = 1).
-template <typename S, int SHAPE, int V, int NY, int PF>
+// SI / SO: storage of src / dst, as for Stencil3DHL; the first sweep's planes
+// stay in registers in the accumulation type.
+template <typename SI, int SHAPE, int V, int NY, int PF, typename SO = SI>
 struct Stencil3DHL2 {
-    using A = typename Storage<S>::acc;
-    static_assert(sizeof(S) == sizeof(A), "F32 / F64 storage");
+    using A = typename Storage<SI>::acc;
+    static_assert(std::is_same<A, typename Storage<SO>::acc>::value, "one accumulation type");
     static_assert(PF >= 1, "prefetch at least one plane");
     static constexpr int R = 1;
     static constexpr int NT = ntaps3(SHAPE, R);
@@ -220,7 +207,7 @@ struct Stencil3DHL2 {
     static constexpr int NM = NY + 2;            // first-sweep rows y0-1 .. y0+NY
     static constexpr int NPI = 3 + PF;           // input planes in the ring
     static constexpr int PERIOD = clcm(NPI, 3);  // ticks until both rings realign
-    static constexpr uint32_t ES = sizeof(S);
+    static constexpr uint32_t ESI = sizeof(SI), ESO = sizeof(SO);
 
     struct PlaneI { A v[NI][V]; };
     struct PlaneM { A v[NM][V]; };
@@ -228,7 +215,7 @@ struct Stencil3DHL2 {
     struct Ctx {
         BufRsrc src, dst;
         int ny, nz, zs;
-        uint32_t sy, sz;
+        uint32_t syi, szi, syo, szo;  // byte strides of src and dst
         int y0;
         int ze;
         bool single;          // one sweep only (first-sweep planes are the output)
@@ -239,34 +226,15 @@ struct Stencil3DHL2 {
 
     template <bool VEC>
     __device__ static __forceinline__ void store_row(const Ctx& c, uint32_t roff, const A (&out)[V]) {
-        if constexpr (VEC) {
-            buf_store<A, V>(c.dst, c.so[0] + roff, out);
-        } else {
-#pragma unroll
-            for (int j = 0; j < V; ++j) {
-                const A e[1] = {out[j]};
-                buf_store<A, 1>(c.dst, c.so[j] + roff, e);
-            }
-        }
+        buf_store_elems<SO, V, VEC>(c.dst, c.so, roff, out);
     }
 
     template <bool VEC>
     __device__ static __forceinline__ void load_plane(const Ctx& c, int z, PlaneI& P) {
-        const uint32_t pz = static_cast<uint32_t>(z) * c.sz + static_cast<uint32_t>(c.y0 - 2) * c.sy;
+        const uint32_t pz = static_cast<uint32_t>(z) * c.szi + static_cast<uint32_t>(c.y0 - 2) * c.syi;
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            const uint32_t off = c.lo + pz + static_cast<uint32_t>(i) * c.sy;
-            if constexpr (VEC) {
-                buf_load<A, V>(c.src, off, P.v[i]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < V; ++j) {
-                    A e[1];
-                    buf_load<A, 1>(c.src, off + static_cast<uint32_t>(j) * ES, e);
-                    P.v[i][j] = e[0];
-                }
-            }
-        }
+        for (int i = 0; i < NI; ++i)
+            buf_load_elems<SI, V, VEC>(c.src, c.lo + pz + static_cast<uint32_t>(i) * c.syi, P.v[i]);
     }
 
     template <int NR>
@@ -310,18 +278,18 @@ struct Stencil3DHL2 {
                 }
             }
             if (c.single && m >= c.zs && m < c.ze) {
-                const uint32_t mz = static_cast<uint32_t>(m) * c.sz;
+                const uint32_t mz = static_cast<uint32_t>(m) * c.szo;
 #pragma unroll
                 for (int i = 1; i <= NY; ++i) {
                     const int y = c.y0 - 1 + i;
-                    if (y < c.ny) store_row<VEC>(c, mz + static_cast<uint32_t>(y) * c.sy, M.v[i]);
+                    if (y < c.ny) store_row<VEC>(c, mz + static_cast<uint32_t>(y) * c.syo, M.v[i]);
                 }
             }
         }
         // second sweep: output plane q, rows y0 .. y0+NY-1
         if (!c.single && q >= c.zs) {
             const bool zin = q >= 1 && q < c.nz - 1;
-            const uint32_t qz = static_cast<uint32_t>(q) * c.sz;
+            const uint32_t qz = static_cast<uint32_t>(q) * c.szo;
 #pragma unroll
             for (int i = 0; i < NY; ++i) {
                 const int y = c.y0 + i;
@@ -338,7 +306,7 @@ struct Stencil3DHL2 {
                     }
                     out[j] = (yin && c.xin[j]) ? acc : A(0);
                 }
-                if (y < c.ny) store_row<VEC>(c, qz + static_cast<uint32_t>(y) * c.sy, out);
+                if (y < c.ny) store_row<VEC>(c, qz + static_cast<uint32_t>(y) * c.syo, out);
             }
         }
     }
@@ -379,30 +347,31 @@ struct Stencil3DHL2 {
 
     // One tile: strip `strip`, y-block `yb`, output planes [zs, ze) of a dense
     // grid that fits grid3d_fits_buffer, two sweeps from src to dst (single: one).
-    __device__ static __forceinline__ void run_tile(const S* __restrict__ src, S* __restrict__ dst,
+    __device__ static __forceinline__ void run_tile(const SI* __restrict__ src, SO* __restrict__ dst,
                                                     const Grid3D& g, bool vec, const A* __restrict__ w,
                                                     int strip, int yb, int zs, int ze, bool single) {
         const int lane = threadIdx.x & (kWave - 1);
-        const uint32_t bytes = static_cast<uint32_t>(g.nx) * static_cast<uint32_t>(g.ny) *
-                               static_cast<uint32_t>(g.nz) * ES;
+        const uint32_t n = static_cast<uint32_t>(g.nx) * static_cast<uint32_t>(g.ny) * static_cast<uint32_t>(g.nz);
         Ctx c;
-        c.src = buf_rsrc(src, bytes);
-        c.dst = buf_rsrc(dst, bytes);
+        c.src = buf_rsrc(src, n * ESI);
+        c.dst = buf_rsrc(dst, n * ESO);
         c.ny = g.ny;
         c.nz = g.nz;
         c.zs = zs;
         c.ze = ze;
         c.single = single;
-        c.sy = static_cast<uint32_t>(g.sy) * ES;
-        c.sz = static_cast<uint32_t>(g.sz) * ES;
+        c.syi = static_cast<uint32_t>(g.sy) * ESI;
+        c.szi = static_cast<uint32_t>(g.sz) * ESI;
+        c.syo = static_cast<uint32_t>(g.sy) * ESO;
+        c.szo = static_cast<uint32_t>(g.sz) * ESO;
         c.y0 = yb * NY;
         const int x0 = strip * OW + (lane - HL) * V;
-        c.lo = static_cast<uint32_t>(x0) * ES;
+        c.lo = static_cast<uint32_t>(x0) * ESI;
         const bool store_lane = lane >= HL && lane < kWave - HL;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
             const int x = x0 + j;
-            c.so[j] = (store_lane && x < g.nx) ? c.lo + static_cast<uint32_t>(j) * ES : kBufOOB;
+            c.so[j] = (store_lane && x < g.nx) ? static_cast<uint32_t>(x) * ESO : kBufOOB;
             c.xin[j] = x >= 1 && x < g.nx - 1;
         }
         if (V == 1 || vec) stream<true>(c, w, zs, ze);

@@ -128,9 +128,17 @@ def test_stencil3d_validates():
     d.radius = 3
     assert f() == L.E_UNSUPPORTED
     d.radius = 1
-    d.storage = L.BF16
-    assert f() == L.E_UNSUPPORTED
-    d.storage = L.F32
+    d.storage = 7
+    assert f() == L.E_INVALID
+    # BF16 / FP16 multi-sweep runs keep their state in FP32: a work area not
+    # declared as two FP32 grids is refused (the state is never rounded)
+    for sto in (L.BF16, L.F16):
+        d.storage, d.steps = sto, 3
+        assert f(work=FAKE_W) == L.E_INVALID
+        assert f() == L.E_INVALID
+    d.flags = 4
+    assert f(work=FAKE_W) == L.E_INVALID                     # unknown flag
+    d.storage, d.steps, d.flags = L.F32, 1, 0
     d.boundary = L.ZERO
     assert f() == L.E_UNSUPPORTED
     assert lib.ssam_jacobi19(L.F32, FAKE_A, FAKE_B, 0, 4, 4, 1, None, None, None) == L.E_INVALID
@@ -186,8 +194,10 @@ def test_batch_plan_validates():
     assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
     b.ndim, b.pitch = 2, 10                  # pitch < width
     assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
-    b.ndim, b.nz, b.storage, b.steps, b.pitch = 3, 8, L.BF16, 1, 0
-    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_UNSUPPORTED
+    b.ndim, b.nz, b.storage, b.steps, b.pitch = 3, 8, 9, 1, 0
+    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
+    b.storage, b.steps = L.BF16, 4           # 3D multi-step without work (FP32 state)
+    assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
     b.storage, b.steps = L.F32, 4            # 3D multi-step without work
     assert lib.ssam_batch_plan_create(arr, 1, ctypes.byref(h)) == L.E_INVALID
     assert lib.ssam_batch_run(None, None, 0) == L.E_INVALID
@@ -204,12 +214,13 @@ def test_microbench_validates():
 
 def test_jacobi19_noalloc_validates_without_a_device():
     """The allocation-free Jacobi entry rejects bad requests before any device
-    work: null pointers, 2-byte storage, and grids whose residual partials would
+    work: null pointers, unknown storage, and grids whose residual partials would
     outgrow its fixed array (131072 tiles)."""
     lib = L.lib()
     res = ctypes.c_void_p(0x20000000)
     assert lib.ssam_jacobi19_noalloc(L.F32, None, FAKE_B, 64, 64, 64, 1, None, res, None) == L.E_INVALID
-    assert lib.ssam_jacobi19_noalloc(L.BF16, FAKE_A, FAKE_B, 64, 64, 64, 1, None, res, None) == L.E_UNSUPPORTED
+    assert lib.ssam_jacobi19_noalloc(9, FAKE_A, FAKE_B, 64, 64, 64, 1, None, res, None) == L.E_INVALID
+    assert lib.ssam_jacobi19_noalloc(L.BF16, FAKE_A, FAKE_B, 64, 64, 64, 2, None, None, None) == L.E_INVALID
     big = ctypes.c_void_p(0x900000000)
     assert lib.ssam_jacobi19_noalloc(L.F32, FAKE_A, big, 1 << 16, 1 << 14, 8, 1, None, res, None) == \
         L.E_UNSUPPORTED
