@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="skip FP64/BF16 side measurements")
     ap.add_argument("--only", default="", help="comma list of suite kernel names (profiling)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic tiling)")
+    ap.add_argument("--tune-file", default="",
+                    help="reuse the suite's tuning from this file when it exists, else tune and save it "
+                         "there (profiled runs then time the same kernels as the run that tuned)")
     ap.add_argument("--ingest-trace", metavar="TRACE_DIR",
                     help="no GPU: summarise a rocprofv3 --kernel-trace run of this bench (the "
                          "timed dispatches of each suite kernel) into profiles/ and exit")
@@ -62,6 +65,9 @@ def parse():
                          "WRITE_SIZE) into per-kernel HBM bytes under profiles/ and exit")
     ap.add_argument("--side", choices=("config0", "config2", "config3", "config4"),
                     help="run only this side measurement and print its JSON (for profiling)")
+    ap.add_argument("--ingest-sq", nargs="+", metavar="ARG",
+                    help="NAME COMMAND DIR [DIR ...]: summarise rocprofv3 --pmc runs of SQ / GRBM "
+                         "counters (one directory per pass) into profiles/NAME.md and exit")
     ap.add_argument("--ingest-stats", nargs=3, metavar=("STATS_DIR", "NAME", "COMMAND"),
                     help="no GPU: summarise a rocprofv3 --kernel-trace --stats run (its "
                          "kernel_stats.csv) into profiles/NAME.md, quoting COMMAND, and exit")
@@ -86,6 +92,86 @@ def ingest_stats(stats_dir: str, name: str, command: str) -> None:
     dst = pathlib.Path(__file__).resolve().parent / "profiles" / f"{name}.md"
     dst.write_text("\n".join(out) + "\n")
     print(dst)
+
+
+def ingest_sq(name: str, command: str, dirs) -> None:
+    """Per-kernel SQ / GRBM counters of rocprofv3 --pmc runs (one directory per
+    counter pass): instruction mix, issue rate and occupancy of every SSAM kernel
+    dispatched after the bench's 16-byte marker copy (all dispatches when the run
+    has no marker).  SQ_WAVE_CYCLES, SQ_WAIT_* and SQ_ACTIVE_* count quad-cycles;
+    GRBM_GUI_ACTIVE sums the 8 XCDs' clocks."""
+    import csv
+    import collections
+
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in dirs:
+        f = next(pathlib.Path(d).rglob("*counter_collection.csv"))
+        rows = list(csv.DictReader(open(f)))
+        mark = next((int(r["Dispatch_Id"]) for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"]))
+                     if "stream_copy" in r["Kernel_Name"] and int(r["Grid_Size"]) <= 256), 0)
+        disp = collections.defaultdict(dict)
+        for r in rows:
+            if int(r["Dispatch_Id"]) <= mark or "ssam" not in r["Kernel_Name"] or "stream_copy" in r["Kernel_Name"] \
+                    or "fill_uniform" in r["Kernel_Name"]:
+                continue
+            disp[(r["Kernel_Name"], r["Dispatch_Id"])][r["Counter_Name"]] = float(r["Counter_Value"])
+        for (k, _), c in disp.items():
+            for cn, v in c.items():
+                per[k.split("(")[0].replace("void ", "")][cn].append(v)
+    cus = 256
+    md = [f"# {name}", "", "Commands (one MI355X, repository root; one counter pass per run):", "", "```",
+          command, "```", "",
+          "Derived per kernel (averages over its dispatches): VALU/clk/CU = SQ_INSTS_VALU / CUs / "
+          "(GRBM_GUI_ACTIVE / 8); waves/SIMD = SQ_WAVE_CYCLES * 4 / (GRBM_GUI_ACTIVE / 8) / CUs / 4; "
+          "wait = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES; FMA share = SQ_INSTS_VALU_FMA_F32 / SQ_INSTS_VALU; "
+          "DPP-fused adds appear in SQ_INSTS_VALU_ADD_F32.", "",
+          "| kernel | dispatches | VALU/clk/CU | waves/SIMD | wait frac | FMA share | ADD_F32 share | SALU/VALU | "
+          "SMEM/VALU | LDS instr/VALU | VMEM rd/VALU |", "|---|---|---|---|---|---|---|---|---|---|---|"]
+
+    def avg(c, k):
+        v = c.get(k)
+        return sum(v) / len(v) if v else None
+
+    rows = []
+    for k, c in per.items():
+        g = avg(c, "GRBM_GUI_ACTIVE")
+        valu = avg(c, "SQ_INSTS_VALU")
+        if not g or not valu:
+            continue
+        clk = g / 8
+        wc = avg(c, "SQ_WAVE_CYCLES")
+        fmt = lambda x: "" if x is None else f"{x:.3f}"
+        ratio = lambda a: None if avg(c, a) is None else avg(c, a) / valu
+        rows.append((valu, f"| `{k[:120]}` | {len(c['SQ_INSTS_VALU'])} | {valu / cus / clk:.2f} | "
+                           f"{fmt(None if wc is None else wc * 4 / clk / cus / 4)} | "
+                           f"{fmt(None if wc is None or avg(c, 'SQ_WAIT_INST_ANY') is None else avg(c, 'SQ_WAIT_INST_ANY') / wc)} | "
+                           f"{fmt(ratio('SQ_INSTS_VALU_FMA_F32'))} | {fmt(ratio('SQ_INSTS_VALU_ADD_F32'))} | "
+                           f"{fmt(ratio('SQ_INSTS_SALU'))} | {fmt(ratio('SQ_INSTS_SMEM'))} | "
+                           f"{fmt(ratio('SQ_INSTS_LDS'))} | {fmt(ratio('SQ_INSTS_VMEM_RD'))} |"))
+    md += [r for _, r in sorted(rows, reverse=True)]
+    dst = ROOT / "profiles" / f"{name}.md"
+    dst.write_text("\n".join(md) + "\n")
+    print("\n".join(md))
+
+
+def suite_kernel(k: str):
+    """(suite name, steps per launch) of an FP32 suite kernel's full name, or None:
+    the register-cache kernel ssam2d_kernel<float, R, SHAPE, V, T, ...> or the
+    LDS-tap form ssam2d_lds_kernel<float, R, SHAPE, ...> (one step)."""
+    import re
+
+    m = re.search(r"ssam2d_kernel<float, (\d+), (\d+), \d+, (\d+),", k)
+    if m:
+        r, shape, t = (int(v) for v in m.groups())
+    else:
+        m = re.search(r"ssam2d_lds_kernel<float, (\d+), (\d+),", k)
+        if not m:
+            return None
+        r, shape, t = int(m.group(1)), int(m.group(2)), 1
+    for n, sh, rr, tt in SUITE:
+        if rr == r and (1 if sh == "box" else 0) == shape:
+            return n, t
+    return None
 
 
 def ingest_trace(trace_dir: str, steps: int, warmup: int) -> None:
@@ -152,7 +238,7 @@ def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
         for r in rows:
             if "stream_copy" in r["Kernel_Name"] and int(r["Grid_Size"]) > 256:
                 copies.append(float(r["Counter_Value"]))
-            elif int(r["Dispatch_Id"]) > mark and "ssam2d_kernel" in r["Kernel_Name"]:
+            elif int(r["Dispatch_Id"]) > mark and "ssam2d_" in r["Kernel_Name"]:
                 timed.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
         return copies, timed
 
@@ -160,17 +246,15 @@ def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
     gib_kib = float(1 << 20)
     fs = gib_kib / (sum(fc) / len(fc))
     ws = gib_kib / (sum(wc) / len(wc))
-    names = {(1 if sh == "box" else 0, r): (n, t) for n, sh, r, t in SUITE}
+    tsuite_of = {n: t for n, _, _, t in SUITE}
     alg = 2 * 4 * N * N
     rows = []
     for k in fetch:
-        m = re.search(r"ssam2d_kernel<float, (\d+), (\d+), \d+, (\d+),", k)
-        if not m or k not in write:
+        sk = suite_kernel(k)
+        if not sk or k not in write:
             continue
-        r, shape, t = (int(v) for v in m.groups())
-        if (shape, r) not in names:
-            continue
-        name, tsuite = names[(shape, r)]
+        name, t = sk
+        tsuite = tsuite_of[name]
         fb = sum(fetch[k]) / len(fetch[k]) * fs * 1024
         wb = sum(write[k]) / len(write[k]) * ws * 1024
         rec = dict(kernel=k.split("(")[0], steps_per_launch=t, launches_per_step=tsuite // t,
@@ -254,6 +338,9 @@ def main():
     if a.ingest_stats:
         ingest_stats(*a.ingest_stats)
         return
+    if a.ingest_sq:
+        ingest_sq(a.ingest_sq[0], a.ingest_sq[1], a.ingest_sq[2:])
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SSAM stencil path runs on a single MI355X here "
               "(multi-GPU slab decomposition is described in DESIGN.md, not run)", file=sys.stderr)
@@ -302,10 +389,17 @@ def main():
     # autotune each kernel's tiling and temporal-blocking degree on these buffers
     # (before the timed region; the library caches the choice per request shape)
     tuning = {}
-    if not a.no_tune:
+    tune_path = pathlib.Path(a.tune_file) if a.tune_file else None
+    if tune_path is not None and tune_path.exists():
+        api.tune_load(str(tune_path))
+        tuning = {k: tuple(v) for k, v in json.loads(tune_path.with_suffix(".json").read_text()).items()}
+    elif not a.no_tune:
         for name, shape, r, t in suite:
             tuning[name] = api.tune2d(src, dst, taps(shape, r),
                                       shape=L.STAR if shape == "star" else L.BOX, steps=t, work=work)
+        if tune_path is not None:
+            api.tune_save(str(tune_path))
+            tune_path.with_suffix(".json").write_text(json.dumps(tuning))
     torch.cuda.synchronize()
     # a 16-byte copy marks the start of the warm-up + timed launches in a kernel
     # trace (bench.py --ingest-trace keeps the dispatches after it)
