@@ -808,7 +808,15 @@ int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* 
     }
     if (seg_rows_2d)
         for (int k = 0; k < kNumKind2D; ++k) seg_rows_2d[k] = plan->seg2d[k];
-    if (variant_3d) *variant_3d = plan->var3d;
+    if (variant_3d) {
+        // the shape the plan's 3D groups run (BF16 / F16 groups: always two sweeps)
+        *variant_3d = plan->var3d;
+        for (const auto& g : plan->groups)
+            if (g.ndim == 3) {
+                *variant_3d = var3d_of(plan, g);
+                if (!two_byte(g.storage)) break;
+            }
+    }
     if (zseg_3d) *zseg_3d = plan->zseg3d;
     return SSAM_OK;
 }

@@ -5,6 +5,7 @@
 #include "stencil2d.hpp"
 #include "stencil2d_lds.hpp"
 #include "stencil2d_mfma.hpp"
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -31,15 +32,22 @@ inline long long resident_waves(const void* kernel) {
 // rounds of `slots` resident waves (so no round holds a handful of stragglers:
 // 4130 tiles on 4096 slots ran a second round for 34 of them), and never below
 // min_seg rows per tile (the 2*T*r halo rows per tile stay a small fraction).
+// When min_seg caps the segment count below the target, the rounds are chosen
+// among those that min_seg still lets fill completely (the shortest tiles then
+// only when the target asks for more than that).
 inline int tile_rows(int64_t height, int nstrips, int waves_per_simd, long long slots, int min_seg) {
-    long long target = static_cast<long long>(waves_per_simd) * 1024;
+    const long long target = static_cast<long long>(waves_per_simd) * 1024;
+    const long long nsegs_max = std::max<long long>(1, height / std::max(1, min_seg));
+    long long nsegs;
     if (slots > 0) {
-        long long rounds = (target + slots / 2) / slots;
-        if (rounds < 1) rounds = 1;
-        target = rounds * slots;
+        const long long rounds = std::max<long long>(1, (target + slots / 2) / slots);
+        const long long full = nsegs_max * nstrips / slots;  // rounds min_seg lets fill
+        // rounded down: at most rounds * slots tiles
+        nsegs = rounds <= full ? rounds * slots / nstrips : nsegs_max;
+    } else {
+        nsegs = target / nstrips;
     }
-    long long nsegs = target / nstrips;  // rounded down: at most `target` tiles
-    if (nsegs < 1) nsegs = 1;
+    nsegs = std::max<long long>(1, std::min(nsegs, nsegs_max));
     long long seg = (height + nsegs - 1) / nsegs;
     if (seg < min_seg) seg = min_seg;
     if (seg > height) seg = height;
