@@ -302,7 +302,11 @@ void ssam_batch_plan_destroy(ssam_batch_plan plan);
  *   which 9..13: device rate in TFLOP/s, 32 waves per CU with 8 independent
  *     chains, timed with HIP events (wall clock) -- 9 FP32 FMA (no packed FP32),
  *     10 FP64 FMA, 11 v_mfma_f32_16x16x4_f32, 12 v_mfma_f32_4x4x1_16b_f32,
- *     13 v_mfma_f64_16x16x4_f64.
+ *     13 v_mfma_f64_16x16x4_f64;
+ *   which 14..19: FP32 FMA operand forms, TFLOP/s of the FMAs -- 14 every operand
+ *     a VGPR, 15 one SGPR tap, 16 sixteen kernel-argument taps, 17 SGPR taps with
+ *     one DPP add per 4 FMAs, 18 VGPR taps with 2 DPP adds per 10 FMAs (the hop's
+ *     issue cost), 19 the same with 2 plain adds.
  * Cycles (0..8) are s_memtime counts.  Synchronises `stream`. */
 int ssam_microbench(int which, double* result, void* stream);
 

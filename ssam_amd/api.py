@@ -362,7 +362,7 @@ MICROBENCH = ("fma_latency_cycles", "dpp_add_latency_cycles", "bpermute_latency_
               "bpermute_per_cycle_per_cu", "lds_roundtrip_per_cycle_per_cu", "mfma_bf16_16x16x32_per_cycle_per_cu",
               "fma_f32_tflops", "fma_f64_tflops", "mfma_f32_16x16x4_tflops", "mfma_f32_4x4x1_tflops",
               "mfma_f64_16x16x4_tflops", "fmac_vgpr_tflops", "fmac_sgpr_tflops", "fmac_16taps_tflops",
-              "fmac_dpp_tflops")
+              "fmac_dpp_tflops", "fmac_vgpr_dpp_tflops", "fmac_vgpr_add_tflops")
 
 
 def microbench(stream: Optional[torch.cuda.Stream] = None) -> dict:

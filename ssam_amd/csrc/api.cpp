@@ -585,6 +585,7 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
     rq.radius = radius;
     rq.taps = taps;
     rq.zseg = env_int("SSAM_ZSEG", 0);
+    rq.form = env_int("SSAM_3D_FORM", 0);
     rq.partials = nullptr;
     rq.partials_cap = 0;
     rq.res_out = nullptr;

@@ -43,7 +43,11 @@ enum MicroOp : int {
     kMbFmacSgprRate = 15,    // v_fmac_f32 acc, s_tap, v_x (one uniform tap)
     kMbFmacTapsRate = 16,    // v_fmac_f32 acc, s_tap[k], v_x over 16 kernel-argument taps
     kMbFmacDppRate = 17,     // 4 FMAs per DPP-fused add (the systolic hop's share)
-    kMbCount = 18
+    // the hop's issue cost among double-rate FMAs (TFLOP/s of the FMAs only): groups
+    // of 10 VGPR-operand FMAs plus 2 DPP-fused adds, or plus 2 plain adds
+    kMbFmacVgprDppRate = 18,
+    kMbFmacVgprAddRate = 19,
+    kMbCount = 20
 };
 
 constexpr int kMbIters = 1024;  // dependent operations per chain
@@ -198,6 +202,36 @@ __global__ __launch_bounds__(256) void mb_fmac_kernel(MbTaps tp, float* sink) {
     sink[blockIdx.x * 256 + threadIdx.x] = s;
 }
 
+// 10 accumulators per lane with VGPR taps; after each group of 10 FMAs, two
+// partials cross one lane (DPP add, the systolic hop) or are added in place.
+constexpr int kMbHopGroup = 10;
+template <int OP>
+__global__ __launch_bounds__(256) void mb_hop_kernel(float a, float* sink) {
+    const int lane = threadIdx.x & (kWave - 1);
+    float x[kMbHopGroup], acc[kMbHopGroup], tv[kMbHopGroup];
+#pragma unroll
+    for (int c = 0; c < kMbHopGroup; ++c) {
+        x[c] = static_cast<float>(lane + c) * 1e-3f;
+        acc[c] = 0.f;
+        tv[c] = a * (1.0f + static_cast<float>(c) * 1e-3f + static_cast<float>(lane) * 1e-7f);
+    }
+    for (int i = 0; i < kMbRateIters; ++i) {
+#pragma unroll
+        for (int c = 0; c < kMbHopGroup; ++c) acc[c] = __builtin_fmaf(tv[c], x[c], acc[c]);
+        if constexpr (OP == kMbFmacVgprDppRate) {
+            acc[0] += from_left(acc[4]);
+            acc[9] += from_right(acc[5]);
+        } else {
+            acc[0] += acc[4];
+            acc[9] += acc[5];
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < kMbHopGroup; ++c) s += acc[c];
+    sink[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 // flops of one chain step of one wave
 template <int OP> constexpr double mb_wave_flops() {
     return OP == kMbFmaF32Rate || OP == kMbFmaF64Rate || OP >= kMbFmacVgprRate ? 2.0 * 64
@@ -282,7 +316,9 @@ template <int OP> int run_rate(double* result, hipStream_t st) {
     for (int rep = 0; rep < 3 && rc == SSAM_OK; ++rep) {  // first run warms the clocks
         float ms = 0.f;
         if (hipEventRecord(e0, st) != hipSuccess) { rc = SSAM_E_HIP; break; }
-        if constexpr (OP >= kMbFmacVgprRate) {
+        if constexpr (OP == kMbFmacVgprDppRate || OP == kMbFmacVgprAddRate) {
+            hipLaunchKernelGGL(mb_hop_kernel<OP>, dim3(blocks), dim3(256), 0, st, 1.0001f, d_sink);
+        } else if constexpr (OP >= kMbFmacVgprRate) {
             MbTaps tp;
             for (int k = 0; k < 16; ++k) tp.t[k] = 1.0001f + 1e-4f * k;
             hipLaunchKernelGGL(mb_fmac_kernel<OP>, dim3(blocks), dim3(256), 0, st, tp, d_sink);
@@ -298,7 +334,8 @@ template <int OP> int run_rate(double* result, hipStream_t st) {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(d_sink);
-    const double flops = static_cast<double>(blocks) * 4 * kMbRateIters * kMbChains * mb_wave_flops<OP>();
+    const int per_iter = OP == kMbFmacVgprDppRate || OP == kMbFmacVgprAddRate ? kMbHopGroup : kMbChains;
+    const double flops = static_cast<double>(blocks) * 4 * kMbRateIters * per_iter * mb_wave_flops<OP>();
     *result = rc == SSAM_OK ? flops / (static_cast<double>(best) * 1e-3) / 1e12 : 0.0;
     return rc;
 }
@@ -326,6 +363,8 @@ extern "C" int ssam_microbench(int which, double* result, void* stream) {
         case kMbFmacVgprRate: return run_rate<kMbFmacVgprRate>(result, st);
         case kMbFmacSgprRate: return run_rate<kMbFmacSgprRate>(result, st);
         case kMbFmacTapsRate: return run_rate<kMbFmacTapsRate>(result, st);
-        default: return run_rate<kMbFmacDppRate>(result, st);
+        case kMbFmacDppRate: return run_rate<kMbFmacDppRate>(result, st);
+        case kMbFmacVgprDppRate: return run_rate<kMbFmacVgprDppRate>(result, st);
+        default: return run_rate<kMbFmacVgprAddRate>(result, st);
     }
 }

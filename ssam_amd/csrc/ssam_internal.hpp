@@ -60,6 +60,7 @@ struct Req3D {
     int radius;
     const double* taps;  // descriptor order (include/ssam.h)
     int zseg;            // output planes per tile, 0 = heuristic
+    int form;            // radius-2 stars: 0 auto, 1 plane ring, 2 z-partial sums
     double* partials;    // device, per-tile residual partials (nullptr: none)
     int partials_cap;
     double* res_out;     // device, the summed residual^2 (or its root, res_sqrt)

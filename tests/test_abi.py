@@ -208,7 +208,7 @@ def test_batch_plan_validates():
 def test_microbench_validates():
     v = ctypes.c_double()
     assert L.lib().ssam_microbench(-1, ctypes.byref(v), None) == L.E_INVALID
-    assert L.lib().ssam_microbench(18, ctypes.byref(v), None) == L.E_INVALID
+    assert L.lib().ssam_microbench(20, ctypes.byref(v), None) == L.E_INVALID
     assert L.lib().ssam_microbench(0, None, None) == L.E_INVALID
 
 

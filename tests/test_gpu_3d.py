@@ -161,3 +161,25 @@ def test_jacobi19_residual_concurrent_threads():
         got = list(ex.map(run, range(len(sizes))))
     for k in range(len(sizes)):
         assert got[k] == [alone[k]] * 4, (sizes[k], got[k], alone[k])
+
+
+@pytest.mark.parametrize("form", ["1", "2"])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("dims,zseg", [((50, 33, 27), "0"), ((260, 9, 10), "3"), ((7, 6, 5), "0"),
+                                        ((130, 20, 40), "5"), ((1, 1, 1), "0"), ((256, 12, 33), "7")])
+def test_radius2_plane_ring_and_z_partial_sums(form, dt, dims, zseg, monkeypatch):
+    """The 13-point star through the plane-ring kernel (SSAM_3D_FORM=1) and the
+    z-partial-sum kernel (2, the default), against the oracle, on ragged grids and
+    short z-segments (tiles that start and end inside the grid)."""
+    nx, ny, nz = dims
+    monkeypatch.setenv("SSAM_3D_FORM", form)
+    monkeypatch.setenv("SSAM_ZSEG", zseg)
+    taps = wl.taps3d("star", 2)
+    coef = wl.uniform(77, len(taps), -1, 1)
+    x = oracle.uniform(12, nx * ny * nz, -1, 1, dt).reshape(nz, ny, nx)
+    src = torch.from_numpy(x).cuda()
+    dst = torch.full_like(src, float("nan"))
+    api.stencil3d(src, dst, coef, shape=L.STAR, radius=2)
+    torch.cuda.synchronize()
+    want = _gold(x, taps, coef, 2, 1)
+    _check(dst.cpu().numpy(), want, _scale(x, taps, coef, 2, 1), TOL[dt], f"form {form} {dims} zseg {zseg}")
