@@ -122,6 +122,8 @@ int stencil2d_run(int storage, int kshape, int R, int steps, int boundary, const
     if (width < 1 || height < 1 || pitch < width) return SSAM_E_INVALID;
     if (width > (1LL << 30) || height > (1LL << 30) || pitch > (1LL << 31))
         return SSAM_E_UNSUPPORTED;
+    // edge tiles address a row (or its FP32 state) through a raw buffer: < 2 GiB
+    if (width * std::max(es, 4) >= (1LL << 31)) return SSAM_E_UNSUPPORTED;
     const int64_t bytes = ((height - 1) * pitch + width) * es;
     if (overlaps(src, bytes, dst, bytes)) return SSAM_E_INVALID;
 

@@ -595,7 +595,7 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
         if (!es) return SSAM_E_INVALID;
         if (it.ndim == 2) {
             if (it.nx < 1 || it.ny < 1 || it.pitch < it.nx || it.nz != 1) return SSAM_E_INVALID;
-            if (it.nx > (1 << 30) || it.ny > (1 << 30)) return SSAM_E_UNSUPPORTED;
+            if (it.nx > (1 << 30) || it.ny > (1 << 30) || it.nx * es >= (1LL << 31)) return SSAM_E_UNSUPPORTED;
             if (it.shape != SSAM_SHAPE_STAR && it.shape != SSAM_SHAPE_BOX) return SSAM_E_INVALID;
             const int64_t bytes = ((it.ny - 1) * it.pitch + it.nx) * es;
             const char* a = static_cast<const char*>(it.src);

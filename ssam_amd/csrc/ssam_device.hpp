@@ -288,6 +288,13 @@ template <typename S, int V> struct RowReg {
             load_strip_vec<S, V>(p, r);
         }
     }
+    // one raw-buffer access of the V elements at byte offset `off`: every dword
+    // at or past the buffer's end reads 0 (the range check of raw buffers is per
+    // dword), and so does an offset at or past 2^31
+    __device__ __forceinline__ void load_buf(BufRsrc rs, uint32_t off) {
+        if constexpr (kPacked) buf_load<uint32_t, N>(rs, off, r);
+        else buf_load<A, V>(rs, off, r);
+    }
     // from V elements in storage form (edge paths)
     __device__ __forceinline__ void set(const S (&e)[V]) {
         if constexpr (kPacked) {

@@ -325,44 +325,56 @@ struct Stencil2D {
         bool inb[V];
     };
 
-    // CHK = false: the wave's whole footprint (rows and columns, halo included) is
+    // CHK = 0: the wave's whole footprint (rows and columns, halo included) is
     // inside the grid and vector-aligned, so rows load with one unconditional
-    // vector access and no boundary rule applies.
+    // vector access and no boundary rule applies.  CHK = 1: an edge tile under
+    // zero padding (or HOLD); CHK = 2: an edge tile clamping to the edge.
     // Row base pointers are wave-uniform (SGPRs); a lane adds its 32-bit byte
     // offset, so loads and stores use the SGPR-base + VGPR-offset address form.
     __device__ static __forceinline__ const char* row_ptr(const Lane& L, int y) {
         return reinterpret_cast<const char*>(L.src) +
                static_cast<long long>(y) * L.p.pitch * static_cast<long long>(sizeof(S));
     }
-    template <bool CHK>
+    template <int CHK>
     __device__ static __forceinline__ void load_row(const Lane& L, int y, Row& dst) {
         if constexpr (!CHK) {
             dst.load_vec(reinterpret_cast<const S*>(row_ptr(L, y) + L.xoff));
-        } else {
-            int yy = y;
-            const bool clamp = L.p.boundary == kClamp;
-            if (clamp) {
-                yy = clampi(y, 0, L.p.height - 1);
-            } else if (y < 0 || y >= L.p.height) {
-                dst.zero();
-                return;
+        } else if constexpr (CHK == 1) {
+            // Zero padding (and HOLD, whose stored cells never read outside the
+            // grid): one raw-buffer access over the row, with no branch.  Rows
+            // outside the grid take an offset past the buffer, columns outside
+            // [0, width) lie past its end (x0 < 0 wraps past 2^31), and both read
+            // 0 -- so the loads of an edge tile stay in flight across ticks like an
+            // interior tile's (element loads under per-lane branches made the
+            // compiler wait for every load at each row).
+            const bool yin = y >= 0 && y < L.p.height;
+            const BufRsrc rs = buf_rsrc(row_ptr(L, yin ? y : 0),
+                                        static_cast<uint32_t>(L.p.width) * static_cast<uint32_t>(sizeof(S)));
+            dst.load_buf(rs, yin ? L.xoff : kBufOOB);
+            if constexpr (Row::kPacked) {
+                // odd width: the dword holding the last element also holds the
+                // element past the row end, so the range check drops it whole;
+                // the last element comes on its own (0 for every other lane)
+                const int k = L.p.width - 1 - L.x0;  // its index in this lane's strip
+                const bool mine = yin && (L.p.width & 1) && k >= 0 && k < V;
+                const uint32_t last = __builtin_amdgcn_raw_buffer_load_b16(
+                    rs, static_cast<int>(mine ? L.xoff + static_cast<uint32_t>(k) * 2u : kBufOOB), 0, 0);
+#pragma unroll
+                for (int n = 0; n < Row::N; ++n) dst.r[n] = mine && k == 2 * n ? last : dst.r[n];
             }
-            const char* row = row_ptr(L, yy);
+        } else {
+            // clamp to the edge: the nearest row, and per column the nearest
+            // column for lanes that reach past an edge
+            const char* row = row_ptr(L, clampi(y, 0, L.p.height - 1));
             if (L.lane_full && L.p.vec_ok) {
                 dst.load_vec(reinterpret_cast<const S*>(row + L.xoff));
             } else {
                 S e[V];
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
-                    const int x = L.x0 + j;
-                    if (clamp) {
-                        const uint32_t o = static_cast<uint32_t>(clampi(x, 0, L.p.width - 1)) *
-                                           static_cast<uint32_t>(sizeof(S));
-                        e[j] = gld<S>(row + o);
-                    } else {
-                        const uint32_t o = L.xoff + static_cast<uint32_t>(j * sizeof(S));
-                        e[j] = L.inb[j] ? gld<S>(row + o) : S{};
-                    }
+                    const uint32_t o = static_cast<uint32_t>(clampi(L.x0 + j, 0, L.p.width - 1)) *
+                                       static_cast<uint32_t>(sizeof(S));
+                    e[j] = gld<S>(row + o);
                 }
                 dst.set(e);
             }
@@ -389,20 +401,43 @@ struct Stencil2D {
                                ok ? L.xoff_out : kBufOOB, words);
     }
 
-    template <bool CHK>
+    // Edge tiles store through a raw buffer over the row with no branch: halo
+    // lanes, rows outside the segment and lanes left of the grid take an offset
+    // past the buffer; dwords past the row end are dropped by the range check,
+    // which works per dword -- so a lane straddling the right edge stores exactly
+    // its in-grid elements, except that a 2-byte element whose dword also holds
+    // the element past the end (odd width) is stored on its own.  Keeping every
+    // store unconditional keeps the memory-counter waits of the loads exact.
+    __device__ static __forceinline__ void store_row_edge(const Lane& L, int y, const A (&v)[V]) {
+        constexpr uint32_t ES = sizeof(SO);
+        const bool ok = L.store_lane && y >= L.ys && y < L.ye;
+        char* row = reinterpret_cast<char*>(L.dst) +
+                    static_cast<long long>(y < 0 ? 0 : y) * L.p.pitch_out * static_cast<long long>(ES);
+        const BufRsrc rs = buf_rsrc(row, static_cast<uint32_t>(L.p.width) * ES);
+        SO tmp[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) tmp[j] = Storage<SO>::store(v[j]);
+        constexpr int W = V * static_cast<int>(ES) / 4;
+        uint32_t words[W];
+        __builtin_memcpy(words, tmp, sizeof(tmp));
+        buf_store<uint32_t, W>(rs, ok ? L.xoff_out : kBufOOB, words);
+        if constexpr (ES == 2) {
+            const int k = L.p.width - 1 - L.x0;  // the last column's index in this strip
+            const bool mine = ok && (L.p.width & 1) && k >= 0 && k < V;
+            uint16_t last = tmp[0].bits;
+#pragma unroll
+            for (int j = 1; j < V; ++j) last = k == j ? tmp[j].bits : last;
+            __builtin_amdgcn_raw_buffer_store_b16(
+                last, rs, static_cast<int>(mine ? L.xoff_out + static_cast<uint32_t>(k) * ES : kBufOOB), 0, 0);
+        }
+    }
+
+    // interior tiles (scalar-tap kernels): one vector store per lane
     __device__ static __forceinline__ void store_row(const Lane& L, int y, const A (&v)[V]) {
         if (!L.store_lane) return;
         char* row = reinterpret_cast<char*>(L.dst) +
                     static_cast<long long>(y) * L.p.pitch_out * static_cast<long long>(sizeof(SO));
-        if (!CHK || (L.lane_full && L.p.vec_ok)) {
-            store_strip_vec<SO, V>(reinterpret_cast<SO*>(row + L.xoff_out), v);
-        } else {
-#pragma unroll
-            for (int j = 0; j < V; ++j)
-                if (L.inb[j])
-                    gst<SO>(row + L.xoff_out + static_cast<uint32_t>(j * sizeof(SO)),
-                            Storage<SO>::store(v[j]));
-        }
+        store_strip_vec<SO, V>(reinterpret_cast<SO*>(row + L.xoff_out), v);
     }
 
     // BF16 / F16: cv holds the window rows converted to FP32, each row converted
@@ -411,7 +446,7 @@ struct Stencil2D {
     using TP = TapPolicy<A, R, SHAPE>;
     using TRegs = TapRegs<A, K, SHAPE>;
 
-    template <bool CHK, int KT>
+    template <int CHK, int KT>
     __device__ static __forceinline__ void tick(const Lane& L, const A* __restrict__ w, const TRegs& tr,
                                                 int yc0, Row (&ring0)[NR0], A (&cv)[NR0][V],
                                                 A (&rings)[NSTAGE_RINGS][NRS][V]) {
@@ -448,38 +483,35 @@ struct Stencil2D {
             if (t < T) {
                 if constexpr (CHK) {
                     // zero outside the grid so the next step sees zero padding
-                    if (q < 0 || q >= L.p.height) {
+                    const bool qin = q >= 0 && q < L.p.height;
 #pragma unroll
-                        for (int j = 0; j < V; ++j) out[j] = 0;
-                    } else if (L.edge) {
-#pragma unroll
-                        for (int j = 0; j < V; ++j) out[j] = L.inb[j] ? out[j] : A(0);
-                    }
+                    for (int j = 0; j < V; ++j) out[j] = qin && L.inb[j] ? out[j] : A(0);
                 }
 #pragma unroll
                 for (int j = 0; j < V; ++j) rings[t - 1][sS(KT, 2 * R)][j] = out[j];
             } else if constexpr (!CHK && TP::mode != kTapSgpr) {
                 store_row_nb(L, q, out);
-            } else if (q >= L.ys && q < L.ye) {
-                if constexpr (CHK && T == 1) {
-                    if (L.p.boundary == kHold) {
-                        // border cells keep their input value
-                        const Row& in = ring0[s0(KT, R)];
-                        const bool row_border = q < R || q >= L.p.height - R;
+            } else if constexpr (CHK) {
+                if constexpr (T == 1) {
+                    // HOLD: border cells keep their input value
+                    const bool hold = L.p.boundary == kHold;
+                    const Row& in = ring0[s0(KT, R)];
+                    const bool row_border = q < R || q >= L.p.height - R;
 #pragma unroll
-                        for (int j = 0; j < V; ++j) {
-                            const int x = L.x0 + j;
-                            const bool border = row_border || x < R || x >= L.p.width - R;
-                            out[j] = border ? in.get(j) : out[j];
-                        }
+                    for (int j = 0; j < V; ++j) {
+                        const int x = L.x0 + j;
+                        const bool border = hold && (row_border || x < R || x >= L.p.width - R);
+                        out[j] = border ? in.get(j) : out[j];
                     }
                 }
-                store_row<CHK>(L, q, out);
+                store_row_edge(L, q, out);
+            } else if (q >= L.ys && q < L.ye) {
+                store_row(L, q, out);
             }
         }
     }
 
-    template <bool CHK, int KT>
+    template <int CHK, int KT>
     __device__ static __forceinline__ void block(const Lane& L, const A* __restrict__ w, const TRegs& tr,
                                                  int yc0, Row (&ring0)[NR0], A (&cv)[NR0][V],
                                                  A (&rings)[NSTAGE_RINGS][NRS][V]) {
@@ -496,7 +528,7 @@ struct Stencil2D {
         }
     }
 
-    template <bool CHK>
+    template <int CHK>
     __device__ static __forceinline__ void stream(const Lane& L, const A* __restrict__ w, int c0,
                                                   int nblk) {
         Row ring0[NR0];
@@ -593,9 +625,11 @@ struct Stencil2D {
         const bool interior = !L.edge && p.vec_ok && y_lo >= 0 && y_hi < p.height &&
                               static_cast<long long>(p.width) * static_cast<long long>(sizeof(SO)) < (1LL << 31);
         if (interior)
-            stream<false>(L, w, c0, nblk);
+            stream<0>(L, w, c0, nblk);
+        else if (p.boundary == kClamp)
+            stream<2>(L, w, c0, nblk);
         else
-            stream<true>(L, w, c0, nblk);
+            stream<1>(L, w, c0, nblk);
     }
 
     __device__ static void run(const S* __restrict__ src, SO* __restrict__ dst, const Params2D& p,

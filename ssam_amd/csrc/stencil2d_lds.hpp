@@ -156,7 +156,7 @@ struct Stencil2DLds : Stencil2D<S, R, SHAPE, V, 1, PF, false, M> {
     static constexpr int NR0 = K + PF + M - 1;  // input ring
     static_assert(PF >= M, "a block's new rows are consumed by later blocks");
 
-    template <bool CHK>
+    template <int CHK>
     __device__ static __forceinline__ void block(const Lane& L, LdsPtr<A> tl, int yc0, Row (&ring0)[NR0]) {
         A X[2 * R + M][V];
 #pragma unroll
@@ -175,23 +175,22 @@ struct Stencil2DLds : Stencil2D<S, R, SHAPE, V, 1, PF, false, M> {
             const int q = yc0 + i - R;
             if constexpr (!CHK) {
                 Base::store_row_nb(L, q, out[i]);
-            } else if (q >= L.ys && q < L.ye) {
-                if (L.p.boundary == kHold) {
-                    // border cells keep their input value (ring slot of row q)
-                    const bool row_border = q < R || q >= L.p.height - R;
+            } else {
+                // HOLD: border cells keep their input value (ring slot of row q)
+                const bool hold = L.p.boundary == kHold;
+                const bool row_border = q < R || q >= L.p.height - R;
 #pragma unroll
-                    for (int j = 0; j < V; ++j) {
-                        const int x = L.x0 + j;
-                        const bool border = row_border || x < R || x >= L.p.width - R;
-                        out[i][j] = border ? X[R + i][j] : out[i][j];
-                    }
+                for (int j = 0; j < V; ++j) {
+                    const int x = L.x0 + j;
+                    const bool border = hold && (row_border || x < R || x >= L.p.width - R);
+                    out[i][j] = border ? X[R + i][j] : out[i][j];
                 }
-                Base::template store_row<CHK>(L, q, out[i]);
+                Base::store_row_edge(L, q, out[i]);
             }
         }
     }
 
-    template <bool CHK>
+    template <int CHK>
     __device__ static __forceinline__ void stream(const Lane& L, LdsPtr<A> tl, int c0, int nblk) {
         Row ring0[NR0];
 #pragma unroll
@@ -238,9 +237,11 @@ struct Stencil2DLds : Stencil2D<S, R, SHAPE, V, 1, PF, false, M> {
         const bool interior = !L.edge && p.vec_ok && y_lo >= 0 && y_hi < p.height &&
                               static_cast<long long>(p.width) * static_cast<long long>(sizeof(S)) < (1LL << 31);
         if (interior)
-            stream<false>(L, tl, c0, nblk);
+            stream<0>(L, tl, c0, nblk);
+        else if (p.boundary == kClamp)
+            stream<2>(L, tl, c0, nblk);
         else
-            stream<true>(L, tl, c0, nblk);
+            stream<1>(L, tl, c0, nblk);
     }
 };
 

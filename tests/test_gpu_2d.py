@@ -276,6 +276,32 @@ def test_storage_precision_multilaunch_fp32_state(storage, tol, r, steps):
     assert e.value.status == L.E_INVALID
 
 
+@pytest.mark.parametrize("storage,tol", [(torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
+@pytest.mark.parametrize("shape,r,steps,form", [(L.STAR, 1, 1, "0"), (L.BOX, 2, 1, "1"), (L.BOX, 5, 1, "1"),
+                                                (L.STAR, 1, 8, "0"), (L.STAR, 2, 4, "0"), (L.STAR, 5, 1, "0")])
+def test_storage_odd_widths_and_pitches(storage, tol, shape, r, steps, form, monkeypatch):
+    """2-byte rows whose last element shares its dword with the element past the
+    row end (odd widths) and rows at odd pitches: the edge tiles' raw-buffer loads
+    and stores must still read and write every in-grid element and nothing else
+    (dst is NaN-filled, padding columns included)."""
+    monkeypatch.setenv("SSAM_2D_FORM", form)
+    f = wl.normalised_taps("star" if shape == L.STAR else "box", r, 79)
+    for i, (h, w) in enumerate([(37, 251), (61, 249), (5, 3), (130, 497), (9, 1), (40, 245)]):
+        x = oracle.uniform(40 + i, h * w, -1, 1, np.float64).reshape(h, w)
+        pad = 1 if i % 2 else 0
+        src = dev(x, pad, storage)
+        full = torch.full((h, w + pad), float("nan"), dtype=storage, device="cuda")
+        dst = full[:, :w]
+        work = torch.empty(2 * h * w, dtype=torch.float32, device="cuda") if steps > 1 else None
+        api.stencil2d(src, dst, f, shape=shape, steps=steps, work=work)
+        torch.cuda.synchronize()
+        want = oracle.conv2d_iter(x, f, steps)
+        got = full.double().cpu().numpy()
+        assert_close(got[:, :w], want, scale_of(x, f, steps), tol, f"{storage} r={r} T={steps} {(h, w)} pad={pad}")
+        if pad:
+            assert np.isnan(got[:, w:]).all(), f"{storage} {(h, w)}: wrote past the row end"
+
+
 # ------------------------------------------------------------------ full size
 def _band_check(x, got, f, rows, dt, steps=1):
     r = f.shape[0] // 2 * steps
