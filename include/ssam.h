@@ -281,7 +281,8 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
  * their own XCD's queue, 0: all waves work through the tiles in table order),
  * whether 3D sweeps alternate direction (odd steps walk the table back to front),
  * the 2D launch form (1: one persistent launch per stencil kind, 0: one for all
- * kinds), the matrix-core routing of BF16/FP16 boxes (1: on), the waves per
+ * kinds), the matrix-core routing of BF16/FP16 boxes (bit 0: 5x5, bit 1: 11x11
+ * boxes on the matrix cores), the waves per
  * SIMD of the 2D and of the 3D persistent grids (0: as many as fit), and the 3D
  * and 2D launch forms (1: an ordinary grid with one tile per wave, the hardware's
  * workgroup dispatch taking the place of the queues; 0: persistent).  Any

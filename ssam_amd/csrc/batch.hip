@@ -74,7 +74,7 @@ struct ssam_batch_plan_s {
     int aff2d = 1, aff3d = 0;  // queue affinity (QueueSet::affinity)
     int alt3d = 1;             // 3D sweeps alternate direction (QueueSet::alternate)
     int split2d = 1;           // 2D: one launch per kind (1) or one for all kinds (0)
-    int mfma2d = 1;            // BF16/FP16 5x5 and 11x11 boxes on the matrix cores
+    int mfma2d = 3;            // BF16/FP16 boxes on the matrix cores: bit 0 5x5, bit 1 11x11
     int wps2d = 0, wps3d = 1;  // persistent grid: waves per SIMD (0: as many as fit)
     int grid2d = 1, grid3d = 1;  // one tile per wave of an ordinary grid (1) or persistent (0)
     int ncounters = 0;
@@ -301,7 +301,7 @@ int build_group2d(Plan* P, Plan::Group& g) {
         const int idx = static_cast<int>(k);
         const int seg = P->seg2d[b.kind];
         // 5x5 and 11x11 boxes in BF16/FP16 go to the matrix cores
-        if (P->mfma2d && (b.kind == k2Box2 || b.kind == k2Box5) && kHalf2D<S> &&
+        if (((b.kind == k2Box2 && (P->mfma2d & 1)) || (b.kind == k2Box5 && (P->mfma2d & 2))) && kHalf2D<S> &&
             ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 2) == 0) {
             b.kind = b.kind == k2Box2 ? k2Box2M : k2Box5M;
             tiles_for_mfma(it, idx, R, seg, &b.pm, &per[k]);
@@ -655,7 +655,7 @@ int ssam_batch_plan_create(const ssam_batch_item* items, int n, ssam_batch_plan*
     for (int k = 0; k < kNumKind2D; ++k) P->seg2d[k] = seg >= 8 ? seg : kDefaultSeg2D;
     P->aff2d = env_int("SSAM_BATCH2D_AFFINITY", 1) != 0;
     P->split2d = env_int("SSAM_BATCH2D_SPLIT", 1) != 0;
-    P->mfma2d = env_int("SSAM_BATCH2D_MFMA", 1) != 0;
+    P->mfma2d = env_int("SSAM_BATCH2D_MFMA", 3) & 3;
     P->wps2d = std::max(0, env_int("SSAM_BATCH2D_WPS", 0));
     P->wps3d = std::max(0, env_int("SSAM_BATCH3D_WPS", 1));
     P->grid2d = env_int("SSAM_BATCH2D_GRID", 1) != 0;
@@ -736,12 +736,13 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     drop_graph(P);  // the tables change; the next run captures anew
     const int reps = 2;
-    bool has2 = false, has3 = false, kinds[kNumKind2D] = {false, false, false, false};
+    bool has2 = false, has3 = false, half2 = false, kinds[kNumKind2D] = {false, false, false, false};
     for (auto& g : P->groups) {
         if (g.ndim == 3) {
             has3 = true;
         } else {
             has2 = true;
+            half2 = half2 || two_byte(g.storage);
             for (int i : g.ids) {
                 const ssam_batch_item& it = P->items[i];
                 kinds[kind2d(it.shape == SSAM_SHAPE_STAR ? kStar : kBox, it.radius)] = true;
@@ -771,13 +772,43 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
         for (size_t f = 0; f < fields.size(); ++f) *fields[f] = best[f];
     };
     if (has2) {
-        // launch form x matrix-core routing x persistent-grid occupancy, then the
-        // tile height per shape (one shape at a time, the others held), then
-        // queue affinity
-        search({&P->split2d, &P->mfma2d, &P->grid2d}, {{0, 1}, {0, 1}, {0, 1}}, kRun2D);
-        if (!P->grid2d) search({&P->wps2d}, {{1, 2, 4, 0}}, kRun2D);
-        for (int k = 0; k < kNumKind2D; ++k)
-            if (kinds[k]) search({&P->seg2d[k]}, {{32, 64, 128, 256}}, kRun2D);
+        // per matrix-core routing (the matrix-core tiles want other heights than
+        // the VALU ones): launch form x persistent-grid occupancy, then the tile
+        // height per shape (one shape at a time, the others held); the faster
+        // routing wins; then queue affinity
+        struct Cfg { int split, mfma, grid, wps, seg[kNumKind2D]; };
+        auto save = [&]() {
+            Cfg c{P->split2d, P->mfma2d, P->grid2d, P->wps2d, {}};
+            for (int k = 0; k < kNumKind2D; ++k) c.seg[k] = P->seg2d[k];
+            return c;
+        };
+        auto load = [&](const Cfg& c) {
+            P->split2d = c.split, P->mfma2d = c.mfma, P->grid2d = c.grid, P->wps2d = c.wps;
+            for (int k = 0; k < kNumKind2D; ++k) P->seg2d[k] = c.seg[k];
+        };
+        const Cfg start = save();
+        Cfg best = start;
+        float best_ms = 1e30f;
+        // (only BF16 / F16 5x5 and 11x11 boxes have a matrix-core form; every
+        // subset of the present ones is a routing)
+        const int present = half2 ? (kinds[k2Box2] ? 1 : 0) | (kinds[k2Box5] ? 2 : 0) : 0;
+        for (int m = 0; m <= 3 && rc == SSAM_OK; ++m) {
+            if ((m & ~present) != 0) continue;
+            load(start);
+            P->mfma2d = present ? m : start.mfma;
+            search({&P->split2d, &P->grid2d}, {{0, 1}, {0, 1}}, kRun2D);
+            if (!P->grid2d) search({&P->wps2d}, {{1, 2, 4, 0}}, kRun2D);
+            for (int k = 0; k < kNumKind2D; ++k)
+                if (kinds[k]) search({&P->seg2d[k]}, {{32, 64, 128, 256}}, kRun2D);
+            float ms = 0.f;
+            if (rc == SSAM_OK) rc = build_all(P);
+            if (rc == SSAM_OK) rc = time_parts(P, st, kRun2D, reps, &ms);
+            if (rc == SSAM_OK && ms < best_ms) {
+                best_ms = ms;
+                best = save();
+            }
+        }
+        load(best);
         search({&P->aff2d}, {{0, 1}}, kRun2D);
     }
     if (has3) {

@@ -135,27 +135,42 @@ template <> struct BufWords<1> { using type = uint32_t; };
 template <> struct BufWords<2> { typedef uint32_t type __attribute__((ext_vector_type(2))); };
 template <> struct BufWords<4> { typedef uint32_t type __attribute__((ext_vector_type(4))); };
 
-// N consecutive 4-byte elements (or N/2 doubles) at byte offset `off`
+// N consecutive 4-byte elements (or N/2 doubles) at byte offset `off`; 8 dwords
+// go as two 4-dword accesses
 template <typename T, int N>
 __device__ __forceinline__ void buf_load(BufRsrc r, uint32_t off, T (&out)[N]) {
     constexpr int W = N * static_cast<int>(sizeof(T)) / 4;
-    static_assert(W == 1 || W == 2 || W == 4, "1, 2 or 4 dwords");
-    typename BufWords<W>::type v;
-    if constexpr (W == 1) v = __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(off), 0, 0);
-    else if constexpr (W == 2) v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(off), 0, 0);
-    else v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(off), 0, 0);
-    __builtin_memcpy(&out[0], &v, sizeof(v));
+    if constexpr (W == 8) {
+        T (&lo)[N / 2] = *reinterpret_cast<T (*)[N / 2]>(&out[0]);
+        T (&hi)[N / 2] = *reinterpret_cast<T (*)[N / 2]>(&out[N / 2]);
+        buf_load<T, N / 2>(r, off, lo);
+        buf_load<T, N / 2>(r, off + 16u, hi);
+    } else {
+        static_assert(W == 1 || W == 2 || W == 4, "1, 2, 4 or 8 dwords");
+        typename BufWords<W>::type v;
+        if constexpr (W == 1) v = __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(off), 0, 0);
+        else if constexpr (W == 2) v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(off), 0, 0);
+        else v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(off), 0, 0);
+        __builtin_memcpy(&out[0], &v, sizeof(v));
+    }
 }
 
 template <typename T, int N>
 __device__ __forceinline__ void buf_store(BufRsrc r, uint32_t off, const T (&in)[N]) {
     constexpr int W = N * static_cast<int>(sizeof(T)) / 4;
-    static_assert(W == 1 || W == 2 || W == 4, "1, 2 or 4 dwords");
-    typename BufWords<W>::type v;
-    __builtin_memcpy(&v, &in[0], sizeof(v));
-    if constexpr (W == 1) __builtin_amdgcn_raw_buffer_store_b32(v, r, static_cast<int>(off), 0, 0);
-    else if constexpr (W == 2) __builtin_amdgcn_raw_buffer_store_b64(v, r, static_cast<int>(off), 0, 0);
-    else __builtin_amdgcn_raw_buffer_store_b128(v, r, static_cast<int>(off), 0, 0);
+    if constexpr (W == 8) {
+        const T (&lo)[N / 2] = *reinterpret_cast<const T (*)[N / 2]>(&in[0]);
+        const T (&hi)[N / 2] = *reinterpret_cast<const T (*)[N / 2]>(&in[N / 2]);
+        buf_store<T, N / 2>(r, off, lo);
+        buf_store<T, N / 2>(r, off + 16u, hi);
+    } else {
+        static_assert(W == 1 || W == 2 || W == 4, "1, 2, 4 or 8 dwords");
+        typename BufWords<W>::type v;
+        __builtin_memcpy(&v, &in[0], sizeof(v));
+        if constexpr (W == 1) __builtin_amdgcn_raw_buffer_store_b32(v, r, static_cast<int>(off), 0, 0);
+        else if constexpr (W == 2) __builtin_amdgcn_raw_buffer_store_b64(v, r, static_cast<int>(off), 0, 0);
+        else __builtin_amdgcn_raw_buffer_store_b128(v, r, static_cast<int>(off), 0, 0);
+    }
 }
 
 // V consecutive elements of storage type S at byte offset `off` of a raw buffer,

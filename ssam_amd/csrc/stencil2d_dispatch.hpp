@@ -66,10 +66,12 @@ template <typename S> constexpr int storage_of() {
 // sweep, and for fused time steps of radius 1 / radius 2 (fewer columns keep the
 // T-stage register rings in VGPRs and leave room for more waves per SIMD: FP32
 // FMAs reach their double rate only when two waves issue together, so FP32 T=8
-// radius 1 at 2 columns ran 10% faster than at 4), and the most steps fused per
-// launch at radius 2.
+// radius 1 at 2 columns ran 10% faster than at 4; FP32 radius 2 up to T=4 runs
+// on 4 columns, which halves the DPP hops per FMA: two T=4 launches of the
+// 9-point star on 8192^2 0.283 -> 0.251 ms), and the most steps fused per launch
+// at radius 2.
 template <typename S> struct Lanes2D;
-template <> struct Lanes2D<float> { static constexpr int V = 4, VT1 = 2, VT2 = 2, TMAX2 = 8; };
+template <> struct Lanes2D<float> { static constexpr int V = 4, VT1 = 2, VT2 = 4, TMAX2 = 8; };
 template <> struct Lanes2D<double> { static constexpr int V = 2, VT1 = 2, VT2 = 2, TMAX2 = 4; };
 template <> struct Lanes2D<bf16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
 template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
@@ -287,7 +289,9 @@ static int dispatch_fused(const Req2D& rq, hipStream_t st) {
                                   : launch2d_instance<S, 2, kStar, VT2, 4, SO>(rq, st);
         case 8:
             if (rq.radius == 1) return launch2d_instance<S, 1, kStar, VT1, 8, SO>(rq, st);
-            if constexpr (LN::TMAX2 >= 8) return launch2d_instance<S, 2, kStar, VT2, 8, SO>(rq, st);
+            // eight radius-2 stages on 4 columns per lane run out of registers:
+            // 2 columns (the tuner prefers two 4-step launches on 4 columns)
+            if constexpr (LN::TMAX2 >= 8) return launch2d_instance<S, 2, kStar, 2, 8, SO>(rq, st);
             return SSAM_E_UNSUPPORTED;
         default:
             return SSAM_E_UNSUPPORTED;

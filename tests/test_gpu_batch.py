@@ -70,14 +70,14 @@ def _check_refs(refs, tol):
             assert e["max_rel"] <= 3e-5, e
 
 
-@pytest.mark.parametrize("split,mfma,grid", [("1", "1", "0"), ("0", "1", "0"), ("1", "0", "1"), ("0", "0", "1"),
-                                             ("1", "1", "1")])
+@pytest.mark.parametrize("split,mfma,grid", [("1", "3", "0"), ("0", "3", "0"), ("1", "0", "1"), ("0", "0", "1"),
+                                             ("1", "3", "1"), ("1", "1", "1"), ("0", "2", "1")])
 @pytest.mark.parametrize("storage,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
 def test_batch_mixed_2d_3d(storage, tol, split, mfma, grid, monkeypatch):
     """The 2D launch forms (one launch per kind / one for all kinds; persistent /
-    one tile per wave of an ordinary grid), with and without the matrix-core 5x5 /
-    11x11 boxes (BF16/FP16 rows of a multiple of 8 elements), against the oracle
-    on every replay."""
+    one tile per wave of an ordinary grid), with the 5x5 and 11x11 boxes routed to
+    the matrix cores or not (bit 0 / bit 1 of SSAM_BATCH2D_MFMA; BF16/FP16 rows of
+    a multiple of 8 elements), against the oracle on every replay."""
     monkeypatch.setenv("SSAM_BATCH2D_SPLIT", split)
     monkeypatch.setenv("SSAM_BATCH2D_MFMA", mfma)
     monkeypatch.setenv("SSAM_BATCH2D_GRID", grid)
@@ -125,7 +125,7 @@ def test_batch_tune_keeps_results():
     assert set(t["seg_rows_2d"].values()) <= {32, 64, 128, 256}
     assert 0 <= t["variant_3d"] <= 5 and t["zseg_3d"] in (16, 32, 64)
     assert t["affinity_2d"] in (0, 1) and t["affinity_3d"] in (0, 1)
-    assert t["split_2d"] in (0, 1) and t["mfma_2d"] in (0, 1)
+    assert t["split_2d"] in (0, 1) and t["mfma_2d"] in (0, 1, 2, 3)
     assert t["wps_2d"] in (0, 1, 2, 4) and t["wps_3d"] in (1, 2, 4) and t["grid_3d"] in (0, 1)
     assert t["grid_2d"] in (0, 1)
     for rep in range(2):
