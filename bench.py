@@ -201,9 +201,11 @@ def ingest_trace(trace_dir: str, steps: int, warmup: int) -> None:
         rows.append((sum(d) / len(d), k, len(d), min(d), max(d), v[-1][1], v[-1][2], v[-1][3]))
     rows.sort(reverse=True)
     md = ["# rocprofv3 kernel trace: 2D SSAM suite, 8192x8192 FP32 (bench workload)", "",
-          "Command (one MI355X, repo root):", "",
+          "Commands (one MI355X, repo root; the first run tunes and saves the choices in T, the "
+          "profiled run reloads them, so it launches the kernels the timed bench launches):", "",
+          f"    python3 bench.py --no-extras --no-cpu-baseline --tune-file <T> --steps 3 --warmup 1",
           f"    rocprofv3 --kernel-trace --stats --output-format csv -d <D> -o run -- python3 bench.py "
-          f"--steps {steps} --warmup {warmup} --no-extras --no-cpu-baseline",
+          f"--steps {steps} --warmup {warmup} --no-extras --no-cpu-baseline --tune-file <T>",
           f"    python3 bench.py --ingest-trace <D> --steps {steps} --warmup {warmup}", "",
           f"Rows average the {warmup} warm-up + {steps} timed steps' dispatches (after the bench's "
           "16-byte marker copy, i.e. after the autotuner's candidates) and the stream copies; "
@@ -265,9 +267,10 @@ def ingest_pmc(fetch_dir: str, write_dir: str) -> None:
         (ROOT / "profiles" / f"traffic_{name}_8192_f32.json").write_text(json.dumps(rec, indent=1) + "\n")
         rows.append((name, rec))
     md = ["# HBM traffic per launch, FP32 suite kernels on 8192x8192 (MI355X)", "",
-          "Commands (two passes, counters in separate runs):", "",
-          "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline",
-          "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline",
+          "Commands (two passes, counters in separate runs; T holds the tuner's choices, saved by "
+          "`python3 bench.py --no-extras --no-cpu-baseline --tune-file <T>`):", "",
+          "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline --tune-file <T>",
+          "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline --tune-file <T>",
           "    python3 bench.py --ingest-pmc <F> <W>", "",
           f"Calibration: the 1 GiB stream copy reads FETCH_SIZE x{fs:.3f} and WRITE_SIZE x{ws:.3f} "
           "of its true bytes (KiB per dispatch); every figure below is scaled by these.", "",
@@ -343,7 +346,7 @@ def main():
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SSAM stencil path runs on a single MI355X here "
-              "(multi-GPU slab decomposition is described in DESIGN.md, not run)", file=sys.stderr)
+              "(the multi-GPU slab decomposition is a design, not run)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -789,7 +792,7 @@ def side_conv(stream, copy_gbps, tune=True):
 
 
 def side_batch(stream):
-    """The uneven batch (BASELINE config 4): 192 2D grids (5-pt star / 5x5 / 11x11
+    """The uneven batch: 192 2D grids (5-pt star / 5x5 / 11x11
     box) plus 64 3D grids x 10 sweeps of the 7-point star, replayed as one
     hipGraph.  Efficiency = the time the same point counts would take at
     single-large-grid throughput (8192^2 per 2D shape; 512^3 x 10 sweeps of the
@@ -935,7 +938,7 @@ def batch_errors(g2, g3, items) -> dict:
 
 def cpu_baseline(suite, taps):
     """The oracle (restated reference gold, OpenMP over rows) on a bounded slab of
-    the same workload: 64 rows x 8192 columns per kernel."""
+    the same workload: 1024 rows x 8192 columns per kernel."""
     import numpy as np
 
     from oracle import oracle

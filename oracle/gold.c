@@ -39,8 +39,8 @@
 /* Seeded generator: splitmix64 (Steele, Lea, Flood 2014; Vigna's reference   */
 /* C).  Element i is the (i+1)-th output of a splitmix64 stream whose state   */
 /* starts at `seed`, so the stream is random-access.  Replaces the reference's */
-/* glibc rand() (common.h:14-17, common.hpp:56-74), which BASELINE-style      */
-/* seeded inputs cannot reproduce.                                            */
+/* glibc rand() (common.h:14-17, common.hpp:56-74), which explicitly          */
+/* seeded, portable inputs cannot reproduce.                                 */
 /* ------------------------------------------------------------------------- */
 static inline uint64_t orc_mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;

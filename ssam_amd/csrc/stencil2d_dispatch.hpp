@@ -62,19 +62,20 @@ template <typename S> constexpr int storage_of() {
                                             : SSAM_F16;
 }
 
-// Per-storage-type geometry: columns per lane (one 16-byte access) for a single
-// sweep, and for fused time steps of radius 1 / radius 2 (fewer columns keep the
-// T-stage register rings in VGPRs and leave room for more waves per SIMD: FP32
-// FMAs reach their double rate only when two waves issue together, so FP32 T=8
-// radius 1 at 2 columns ran 10% faster than at 4; FP32 radius 2 up to T=4 runs
-// on 4 columns, which halves the DPP hops per FMA: two T=4 launches of the
-// 9-point star on 8192^2 0.283 -> 0.251 ms), and the most steps fused per launch
-// at radius 2.
+// Per-storage-type geometry: columns per lane for a single sweep (V: one 16-byte
+// access; VN: the narrower strip of 2-byte stars of radius >= 4, boxes of radius
+// >= 3 and batch tiles), and for fused time steps of radius 1 / radius 2 (VT1 /
+// VT2), and the most steps fused per launch at radius 2.  Fused steps: fewer
+// columns keep the T-stage register rings in VGPRs with room for more waves per
+// SIMD (FP32 FMAs reach their double rate only when two waves issue together):
+// radius 1 at 2 columns (FP32 T=8 10% faster than at 4, BF16 20%); radius 2 up
+// to T=4 at 4 columns, which halves the DPP hops per FMA (two T=4 launches of the
+// 9-point star on 8192^2: FP32 0.283 -> 0.251 ms, BF16 0.251 -> 0.235).
 template <typename S> struct Lanes2D;
-template <> struct Lanes2D<float> { static constexpr int V = 4, VT1 = 2, VT2 = 4, TMAX2 = 8; };
-template <> struct Lanes2D<double> { static constexpr int V = 2, VT1 = 2, VT2 = 2, TMAX2 = 4; };
-template <> struct Lanes2D<bf16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
-template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VT1 = 4, VT2 = 2, TMAX2 = 8; };
+template <> struct Lanes2D<float> { static constexpr int V = 4, VN = 4, VT1 = 2, VT2 = 4, TMAX2 = 8; };
+template <> struct Lanes2D<double> { static constexpr int V = 2, VN = 2, VT1 = 2, VT2 = 2, TMAX2 = 4; };
+template <> struct Lanes2D<bf16_t> { static constexpr int V = 8, VN = 4, VT1 = 2, VT2 = 4, TMAX2 = 8; };
+template <> struct Lanes2D<f16_t> { static constexpr int V = 8, VN = 4, VT1 = 2, VT2 = 4, TMAX2 = 8; };
 
 
 // Loop form per instance.  RING (no register moves): the body is a full ring
@@ -323,7 +324,7 @@ template <typename S, int SHAPE> int dispatch_shape(const Req2D& rq, hipStream_t
     // kernel (the matrix-core form takes them by default) use 4, whose ring and
     // resident taps fit the register file
     if constexpr (sizeof(S) == 2) {
-        if (rq.radius >= 3) return dispatch_radius<S, SHAPE, Lanes2D<S>::VT1, 1>(rq, st);
+        if (rq.radius >= 3) return dispatch_radius<S, SHAPE, Lanes2D<S>::VN, 1>(rq, st);
         if (rq.radius == 1) return launch2d_instance<S, 1, SHAPE, Lanes2D<S>::V, 1>(rq, st);
         return launch2d_instance<S, 2, SHAPE, Lanes2D<S>::V, 1>(rq, st);
     } else {
@@ -334,7 +335,7 @@ template <typename S, int SHAPE> int dispatch_shape(const Req2D& rq, hipStream_t
 
 template <typename S>
 int dispatch2d(const Req2D& rq, hipStream_t st) {
-    constexpr int V = Lanes2D<S>::V, VT1 = Lanes2D<S>::VT1;
+    constexpr int V = Lanes2D<S>::V, VN = Lanes2D<S>::VN;
     if (rq.out_storage != storage_of<S>()) return dispatch_mixed<S>(rq, st);
     if (rq.steps == 1) {
         // matrix cores: by default from radius 2 (where they beat the VALU kernels
@@ -351,9 +352,9 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
                     case 1: return launch2d_instance<S, 1, kStar, V, 1>(rq, st);
                     case 2: return launch2d_instance<S, 2, kStar, V, 1>(rq, st);
                     case 3: return launch2d_instance<S, 3, kStar, V, 1>(rq, st);
-                    case 4: return launch2d_instance<S, 4, kStar, VT1, 1>(rq, st);
-                    case 5: return launch2d_instance<S, 5, kStar, VT1, 1>(rq, st);
-                    case 6: return launch2d_instance<S, 6, kStar, VT1, 1>(rq, st);
+                    case 4: return launch2d_instance<S, 4, kStar, VN, 1>(rq, st);
+                    case 5: return launch2d_instance<S, 5, kStar, VN, 1>(rq, st);
+                    case 6: return launch2d_instance<S, 6, kStar, VN, 1>(rq, st);
                     default: return SSAM_E_UNSUPPORTED;
                 }
             } else {

@@ -123,7 +123,7 @@ def write_kats(coeffs: dict) -> list[str]:
             save(f"kat_{name}_{np.dtype(dt).name}", src=x.astype(dt), filt=f.astype(dt),
                  out=oracle.ref_conv2d(x.astype(dt), f.astype(dt)))
     # integer convolution taps in [-10, 10] (conv2D.cu:115-119 draws rand()%21-10), over
-    # the filter sizes of BASELINE config 3 and the reference's range up to 28
+    # the 8192^2 convolution workload's filter sizes and the reference's range up to 28
     # (conv2D.cu:186-212); 255 * 10 * 28^2 < 2^24, so every FP32 sum is exact
     for k in (2, 3, 5, 7, 9, 15, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28):
         f = np.floor(wl.uniform(100 + k, k * k, -10.0, 11.0)).reshape(k, k)
