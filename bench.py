@@ -549,7 +549,11 @@ def side_runs(suite, taps, stream, per_f32, tune=True):
     for label, dt in (("fp64", torch.float64), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
         src = x64.to(dt)
         dst = torch.empty_like(src)
-        work = torch.empty_like(src)  # FP64 radius-2 fuses 4 steps per launch
+        # runs over more than one launch (FP64 radius 2 fuses 4 steps; the tuner may
+        # split BF16 / FP16 runs too) chain through work: the storage type for FP64,
+        # two FP32 grids for 2-byte storage, whose running state stays FP32
+        work = torch.empty_like(src) if dt == torch.float64 else \
+            torch.empty(2 * n * n, dtype=torch.float32, device="cuda")
         kern = []
         for i, (name, shape, r, t) in enumerate(suite):
             f = taps(shape, r)
