@@ -373,11 +373,15 @@ def main():
     def taps(shape, r):
         return wl.normalised_taps(shape, r, seed=1000 + 10 * r + (shape == "star"))
 
+    # the host-side taps of every kernel, generated once (like the grids, inputs
+    # prepared before the timed region)
+    tapset = [taps(shape, r) for (_, shape, r, _) in suite]
+
     def run_suite(src, dst, events=None):
         for i, (name, shape, r, t) in enumerate(suite):
             if events is not None:
                 events[i][0].record(stream)
-            api.stencil2d(src, dst, taps(shape, r), shape=L.STAR if shape == "star" else L.BOX,
+            api.stencil2d(src, dst, tapset[i], shape=L.STAR if shape == "star" else L.BOX,
                           steps=t, work=work, stream=stream)
             if events is not None:
                 events[i][1].record(stream)

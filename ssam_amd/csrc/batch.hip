@@ -301,8 +301,10 @@ int build_group2d(Plan* P, Plan::Group& g) {
         const int idx = static_cast<int>(k);
         const int seg = P->seg2d[b.kind];
         // 5x5 and 11x11 boxes in BF16/FP16 go to the matrix cores
+        // (matrix-core edge tiles address the grid through one raw buffer: < 2 GiB)
         if (((b.kind == k2Box2 && (P->mfma2d & 1)) || (b.kind == k2Box5 && (P->mfma2d & 2))) && kHalf2D<S> &&
-            ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 2) == 0) {
+            ((reinterpret_cast<uintptr_t>(it.src) | reinterpret_cast<uintptr_t>(it.dst)) % 2) == 0 &&
+            ((it.ny - 1) * it.pitch + it.nx) * 2 < (1LL << 31)) {
             b.kind = b.kind == k2Box2 ? k2Box2M : k2Box5M;
             tiles_for_mfma(it, idx, R, seg, &b.pm, &per[k]);
             continue;

@@ -68,9 +68,6 @@ struct ParamsM {
 
 constexpr int kWavesPerBlockM = 4;
 
-// 16 zero bytes: the load target of input chunks outside the grid.
-static __device__ __align__(16) const uint32_t kZeroChunk[4] = {0u, 0u, 0u, 0u};
-
 // Rows shifted up by M inside each 16-lane row group: lane i gets x from lane
 // i+M while i+M < 16, else y from lane i+M-16 (x = rows b, y = rows b+1).
 template <int M> __device__ __forceinline__ uint32_t shift_rows(uint32_t x, uint32_t y) {
@@ -110,38 +107,46 @@ struct Mfma2D {
         int x0;        // first output column of the strip
         int ys, ye;    // output rows [ys, ye) of the tile
         int blast;     // last row block any output row needs
+        uint32_t gbytes;  // bytes from the grid's first element to past its last
     };
 
     // Row block b (rows ys - R + 16 b + i) of operand a = t*NC + c (tile t, window
     // c): 8 columns from x0 + 16t - XS + 32c + 8g.
-    // Edge tiles (CHK): rows and 8-column chunks wholly outside the grid read a
-    // zero chunk (one branch-free 16-byte load for every lane); only a chunk
-    // straddling the right edge loads its columns one by one.
+    // Edge tiles (CHK): one raw-buffer access over the whole grid per chunk, with
+    // no branch (per-lane branches around element loads made the compiler wait
+    // for every load in flight).  Rows outside the grid and chunks left of it or
+    // past the right edge take an offset past the buffer and read 0; a chunk
+    // straddling the right edge reads on into the row's padding or the next row
+    // (inside the buffer) and its columns past the edge are cleared.  The dword
+    // holding the grid's very last element (odd width) also holds the element
+    // past the buffer's end, so the range check drops it: that element comes on
+    // its own.  The grid spans < 2 GiB (mfma_eligible).
     template <bool CHK>
     __device__ static __forceinline__ u32x4 load_block(const Lane& L, int b, int a) {
         const int r = L.ys - R + 16 * min(b, L.blast) + L.i;
         const int c = L.x0 + 16 * (a / NC) - XS + 32 * (a % NC) + 8 * L.g;
-        const char* row = L.src + static_cast<long long>(r) * L.p.pitch * static_cast<long long>(sizeof(S));
         if constexpr (!CHK) {
+            const char* row = L.src + static_cast<long long>(r) * L.p.pitch * static_cast<long long>(sizeof(S));
             return gld<u32x4_row>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)));
         } else {
-            const bool rok = r >= 0 && r < L.p.height;
-            const bool inside = c >= 0 && c + 8 <= L.p.width;
-            const void* a = (rok && inside)
-                                ? static_cast<const void*>(row + static_cast<long long>(c) * static_cast<long long>(sizeof(S)))
-                                : static_cast<const void*>(kZeroChunk);
-            u32x4 v = gld<u32x4_row>(a);
-            if (rok && !inside && c < L.p.width && c + 8 > 0) {
-                uint32_t h[8];
+            const bool ok = r >= 0 && r < L.p.height && c >= 0 && c < L.p.width;
+            const uint32_t off = ok ? (static_cast<uint32_t>(r) * static_cast<uint32_t>(L.p.pitch) +
+                                       static_cast<uint32_t>(c)) * 2u
+                                    : kBufOOB;
+            const BufRsrc rs = buf_rsrc(L.src, L.gbytes);
+            uint32_t w[4];
+            buf_load<uint32_t, 4>(rs, off, w);
+            const int k = L.p.width - 1 - c;  // the row's last column within this chunk
+            const bool last_of_grid = ok && r == L.p.height - 1 && (L.p.width & 1) && k < 8;
+            const uint32_t lastv = __builtin_amdgcn_raw_buffer_load_b16(
+                rs, static_cast<int>(last_of_grid ? off + static_cast<uint32_t>(k) * 2u : kBufOOB), 0, 0);
+            u32x4 v;
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int x = c + e;
-                    h[e] = (x >= 0 && x < L.p.width)
-                               ? static_cast<uint32_t>(gld<uint16_t>(row + static_cast<long long>(x) * 2))
-                               : 0u;
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) v[q] = h[2 * q] | (h[2 * q + 1] << 16);
+            for (int q = 0; q < 4; ++q) {
+                // columns c+2q (low half) and c+2q+1 (high half) past the edge read 0
+                const uint32_t lo = 2 * q <= k ? (w[q] & 0xffffu) : 0u;
+                const uint32_t hi = 2 * q + 1 <= k ? (w[q] & 0xffff0000u) : 0u;
+                v[q] = last_of_grid && k == 2 * q ? lastv : (lo | hi);
             }
             return v;
         }
@@ -207,13 +212,24 @@ struct Mfma2D {
         const int x = L.x0 + 16 * t + j;
         // interior tiles: ye - ys is a whole number of 16-row steps, so every row
         // is stored unconditionally (straight-line code keeps the memory counter
-        // waits exact, so prefetched row blocks stay in flight)
+        // waits exact, so prefetched row blocks stay in flight); edge tiles store
+        // through a raw buffer over the grid, outputs outside the tile taking an
+        // offset past it (no branch either)
+        [[maybe_unused]] BufRsrc rs;
+        if constexpr (CHK) rs = buf_rsrc(L.dst, L.gbytes);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int y = y0 + 4 * rg + q;
-            if (CHK && (y >= L.ye || x >= L.p.width)) continue;
-            char* p = L.dst + (static_cast<long long>(y) * L.p.pitch + x) * static_cast<long long>(sizeof(S));
-            gst<S>(p, Storage<S>::store(d[q]));
+            if constexpr (CHK) {
+                const bool ok = y < L.ye && x < L.p.width;
+                const uint32_t off = ok ? (static_cast<uint32_t>(y) * static_cast<uint32_t>(L.p.pitch) +
+                                           static_cast<uint32_t>(x)) * 2u
+                                        : kBufOOB;
+                __builtin_amdgcn_raw_buffer_store_b16(Storage<S>::store(d[q]).bits, rs, static_cast<int>(off), 0, 0);
+            } else {
+                char* p = L.dst + (static_cast<long long>(y) * L.p.pitch + x) * static_cast<long long>(sizeof(S));
+                gst<S>(p, Storage<S>::store(d[q]));
+            }
         }
     }
 
@@ -286,6 +302,7 @@ struct Mfma2D {
         L.ye = min(p.height, L.ys + p.seg_rows);
         // output rows up to ye-1 need input rows up to ye-1+R: block index of that row
         L.blast = (L.ye - 1 + R - (L.ys - R)) / 16;
+        L.gbytes = static_cast<uint32_t>(((static_cast<long long>(p.height) - 1) * p.pitch + p.width) * 2);
         u32x4 bh[K * NC], bl[K * NC];
         build_taps(w, L.i, L.g, bh, bl);
         const bool interior = L.x0 - XS >= 0 && L.x0 + OW - 16 - XS + 32 * NC <= p.width && L.ys - R >= 0 &&
@@ -382,8 +399,10 @@ template <typename S> bool mfma_eligible(const Req2D& rq) {
         // taps are never split: FP16 taps rounded to 11 bits add at most 2^-11 of
         // sum |w x| to the 2^-11 of the FP16 output rounding (inside 1e-3)
         const uintptr_t a = reinterpret_cast<uintptr_t>(rq.src) | reinterpret_cast<uintptr_t>(rq.dst);
+        // (edge tiles address the grid through one raw buffer: under 2 GiB)
+        const long long bytes = ((rq.height - 1) * rq.pitch + rq.width) * 2;
         return rq.shape == kBox && rq.steps == 1 && rq.boundary == kZero && rq.radius >= 1 &&
-               rq.radius <= 9 && a % 2 == 0;
+               rq.radius <= 9 && a % 2 == 0 && bytes < (1LL << 31);
     }
 }
 
