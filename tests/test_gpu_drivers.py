@@ -65,9 +65,9 @@ def test_2dstencil_generates_data_without_image(rundir):
 
 
 @pytest.mark.parametrize("fn", ["7pt", "13pt", "poisson"])
-@pytest.mark.parametrize("dtype", ["float", "double"])
-def test_3dstencil_driver(rundir, fn, dtype):
-    out = _run(rundir, "ssai-3dstencil", fn, dtype, 48)
+@pytest.mark.parametrize("dtype,n", [("float", 48), ("double", 48), ("float", 200), ("double", 130)])
+def test_3dstencil_driver(rundir, fn, dtype, n):
+    out = _run(rundir, "ssai-3dstencil", fn, dtype, n)
     assert "use MI355X GPU" in out and "[Test] Max Error" in out
     rms = float(re.search(r"RMS Error : ([-+0-9.eE]+)", out).group(1))
     assert rms <= 1e-5, out
