@@ -225,6 +225,23 @@ def test_parity_temporal(r, steps, dt):
         assert_close(got, want, scale_of(x, f, steps), TOL[dt] * steps, f"T={steps} {(h, w)}")
 
 
+@pytest.mark.parametrize("r", [1, 2])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_temporal_interior_tiles_exact(r, dt):
+    # Eight fused steps on a grid with interior tiles (whose first blocks leave
+    # out the steps no output needs yet) and edge tiles, integer data and 0/1
+    # taps: every partial sum is an integer below 2^24, so the result must equal
+    # the oracle's exactly.
+    f = np.zeros((2 * r + 1, 2 * r + 1), dtype=dt)
+    f[r, r] = f[0, r] = f[2 * r, r] = f[r, 0] = f[r, 2 * r] = 1
+    rng = np.random.default_rng(17 + r)
+    for (h, w) in [(700, 530), (333, 1000)]:
+        x = rng.integers(0, 3, size=(h, w)).astype(dt)
+        got = run2d(x, f, L.STAR, steps=8)
+        want = oracle.conv2d_iter(x, f, 8)
+        assert np.array_equal(got, want), f"r={r} {(h, w)}: max diff {np.max(np.abs(got - want))}"
+
+
 def test_box_multistep_chains_single_sweeps():
     f = wl.normalised_taps("box", 1, 5)
     x = oracle.uniform(9, 70 * 90, -1, 1, np.float64).reshape(70, 90)
