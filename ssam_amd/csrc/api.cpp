@@ -588,6 +588,7 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
     rq.taps = taps;
     rq.zseg = env_int("SSAM_ZSEG", 0);
     rq.form = env_int("SSAM_3D_FORM", 0);
+    rq.sweeps = 1;
     rq.partials = nullptr;
     rq.partials_cap = 0;
     rq.res_out = nullptr;
@@ -607,16 +608,23 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
             if (rc != SSAM_OK) return rc;
         }
     }
-    // launches: two sweeps each where the temporal form applies (radius 1, grid
-    // under 2 GiB also as the FP32 state of a 2-byte chain, SSAM_3D_T != 1), the
-    // rest one sweep; a residual request runs the last sweep on its own (the
-    // single-sweep kernel forms the residual).
+    // launches: several sweeps each where the temporal forms apply (radius 1, grid
+    // under 2 GiB also as the FP32 state of a 2-byte chain): three for the
+    // 7-point star, two for the 19-point stencil, at most SSAM_3D_T (1, 2 or 3);
+    // four sweeps go as 2+2 rather than 3+1, the rest one sweep; a residual
+    // request runs the last sweep on its own (the single-sweep kernel forms the
+    // residual).
     Req3D probe = rq;
     if (two_byte) probe.out_storage = SSAM_F32;
-    const bool t2 = steps >= 2 && two_sweeps_ok(two_byte ? SSAM_F32 : storage, probe) &&
-                    env_int("SSAM_3D_T", 2) != 1;
+    const int tmax = env_int("SSAM_3D_T", 3);
+    const bool t2 = steps >= 2 && two_sweeps_ok(two_byte ? SSAM_F32 : storage, probe) && tmax != 1;
+    const bool t3 = t2 && tmax >= 3 && rq.shape == 0;
     const int tail = want_res ? 1 : 0;
-    auto sweeps_of = [&](int left) { return (t2 && left - tail >= 2) ? 2 : 1; };
+    auto sweeps_of = [&](int left) {
+        const int l = left - tail;
+        if (t3 && l >= 3 && l != 4) return 3;
+        return (t2 && l >= 2) ? 2 : 1;
+    };
     int nl = 0;
     for (int left = steps; left > 0; left -= sweeps_of(left)) ++nl;
     // F32 / F64: launches ping-pong through work so the last one writes dst;
@@ -659,7 +667,8 @@ int stencil3d_run(int storage, int shape, int radius, const double* taps, int st
                 rq.res_out = scratch;
             }
         }
-        const int rc = n == 2 ? dispatch3d_two_sweeps(in_storage, rq, st) : dispatch3d(in_storage, rq, st);
+        rq.sweeps = n;
+        const int rc = n >= 2 ? dispatch3d_two_sweeps(in_storage, rq, st) : dispatch3d(in_storage, rq, st);
         if (rc != SSAM_OK) {
             // launches already queued may use the scratch: let them finish first
             if (want_res && !fixed) (void)hipStreamSynchronize(st);

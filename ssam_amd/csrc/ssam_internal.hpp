@@ -61,6 +61,7 @@ struct Req3D {
     const double* taps;  // descriptor order (include/ssam.h)
     int zseg;            // output planes per tile, 0 = heuristic
     int form;            // radius-2 stars: 0 auto, 1 plane ring, 2 z-partial sums
+    int sweeps;          // sweeps per launch of dispatch3d_two_sweeps: 2, or 3 (7-point star)
     double* partials;    // device, per-tile residual partials (nullptr: none)
     int partials_cap;
     double* res_out;     // device, the summed residual^2 (or its root, res_sqrt)
@@ -70,7 +71,8 @@ struct Req3D {
 
 // One sweep from `storage` (src) to rq.out_storage (dst).
 int dispatch3d(int storage, const Req3D& rq, hipStream_t st);
-// Two sweeps in one launch (radius 1, grid under 2 GiB); SSAM_E_UNSUPPORTED else.
+// Two sweeps in one launch (radius 1, grid under 2 GiB), or three (rq.sweeps == 3,
+// the 7-point star); SSAM_E_UNSUPPORTED else.
 bool two_sweeps_ok(int storage, const Req3D& rq);
 int dispatch3d_two_sweeps(int storage, const Req3D& rq, hipStream_t st);
 int partials_needed(int storage, const Req3D& rq);

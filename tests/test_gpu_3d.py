@@ -113,11 +113,12 @@ def test_jacobi19_and_residual(dt):
 @pytest.mark.parametrize("shape", ["star", "face_edge"])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("dims", [(50, 33, 27), (260, 9, 10), (7, 6, 5), (130, 20, 40), (1, 1, 1)])
-@pytest.mark.parametrize("steps", [2, 3, 4])
+@pytest.mark.parametrize("steps", [2, 3, 4, 5, 7])
 def test_two_sweeps_per_launch_bit_exact(shape, dt, dims, steps, monkeypatch):
     """Multi-sweep runs launch two sweeps at a time (temporal blocking) for
-    radius-1 stencils; the result equals one sweep per launch bit for bit (same
-    arithmetic per point) and the oracle within the per-sweep tolerance."""
+    radius-1 stencils, three for the 7-point star; the result equals one sweep
+    per launch bit for bit (same arithmetic per point) and the oracle within the
+    per-sweep tolerance."""
     nx, ny, nz = dims
     taps = wl.taps3d(shape, 1)
     coef = wl.uniform(steps + 40, len(taps), -0.5, 0.5)
@@ -125,16 +126,39 @@ def test_two_sweeps_per_launch_bit_exact(shape, dt, dims, steps, monkeypatch):
     src = torch.from_numpy(x).cuda()
     code = L.FACE_EDGE if shape == "face_edge" else L.STAR
     outs = []
-    for t in ("2", "1"):
+    for t in ("3", "2", "1"):
         monkeypatch.setenv("SSAM_3D_T", t)
         dst = torch.full_like(src, float("nan"))
         work = torch.full_like(src, float("nan"))
         api.stencil3d(src, dst, coef, shape=code, radius=1, steps=steps, work=work)
         torch.cuda.synchronize()
         outs.append(dst.cpu().numpy())
-    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
+    assert np.array_equal(outs[1], outs[2])
     want = _gold(x, taps, coef, 1, steps)
     _check(outs[0], want, _scale(x, taps, coef, 1, steps), TOL[dt] * steps, f"{shape} {dims} x{steps}")
+
+
+@pytest.mark.parametrize("zseg", ["1", "2", "5", "64"])
+@pytest.mark.parametrize("steps", [3, 6])
+def test_three_sweeps_short_z_segments(zseg, steps, monkeypatch):
+    """Three sweeps per launch on z-segments from 1 plane up (each tile runs 4
+    fill ticks before its first output plane): bit-equal to one sweep per launch."""
+    nx, ny, nz = 70, 21, 30
+    taps = wl.taps3d("star", 1)
+    coef = wl.uniform(steps + 90, len(taps), -0.5, 0.5)
+    x = oracle.uniform(23, nx * ny * nz, -1, 1, np.float32).reshape(nz, ny, nx)
+    src = torch.from_numpy(x).cuda()
+    monkeypatch.setenv("SSAM_ZSEG", zseg)
+    outs = []
+    for t in ("3", "1"):
+        monkeypatch.setenv("SSAM_3D_T", t)
+        dst = torch.full_like(src, float("nan"))
+        work = torch.full_like(src, float("nan"))
+        api.stencil3d(src, dst, coef, shape=L.STAR, radius=1, steps=steps, work=work)
+        torch.cuda.synchronize()
+        outs.append(dst.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
 
 
 def test_jacobi19_residual_concurrent_threads():

@@ -77,19 +77,21 @@ def test_single_sweep_vs_fp64(storage, tol, shape, r, dims):
 @pytest.mark.parametrize("steps", [2, 3, 10])
 @pytest.mark.parametrize("dims", [(50, 33, 27), (130, 20, 40), (7, 6, 5)])
 def test_multi_sweep_state_stays_fp32(storage, tol, shape, r, steps, dims, monkeypatch):
-    """Several sweeps: one or two per launch give the same result bit for bit
-    (the FP32 state between launches equals the FP32 value the fused launch keeps
-    in registers), within the bar at every step count, and strictly closer to the
-    gold than the same sweeps run one launch at a time through the storage type
-    (state rounded every sweep)."""
+    """Several sweeps: one, two or three per launch give the same result bit for
+    bit (the FP32 state between launches equals the FP32 value the fused launch
+    keeps in registers), within the bar at every step count, and strictly closer
+    to the gold than the same sweeps run one launch at a time through the storage
+    type (state rounded every sweep)."""
     nx, ny, nz = dims
     taps = wl.taps3d(shape, r)
     coef = wl.uniform(r + 70, len(taps), 0, 1)
     coef = coef / coef.sum()
     x = oracle.uniform(steps + 41, nx * ny * nz, -1, 1, np.float64).reshape(nz, ny, nx)
-    fused = _run(x, storage, coef, shape, r, steps, "2", monkeypatch)
+    fused = _run(x, storage, coef, shape, r, steps, "3", monkeypatch)
+    fused2 = _run(x, storage, coef, shape, r, steps, "2", monkeypatch)
     single = _run(x, storage, coef, shape, r, steps, "1", monkeypatch)
     assert np.array_equal(fused, single)
+    assert np.array_equal(fused2, single)
     want, sc = _gold(x, taps, coef, r, steps)
     e = _check(fused, want, sc, tol, f"{storage} {shape} r={r} x{steps} {dims}", storage)
     # the same sweeps with the state rounded to the storage type between them
