@@ -62,6 +62,7 @@ SOURCES = [
     "batch3d_t2.hip",
     "batch3d_t2_bf16.hip",
     "batch3d_t2_f16.hip",
+    "batch3d_t3.hip",
     "microbench.hip",
 ]
 

@@ -93,11 +93,17 @@ bool two_byte(int storage) { return storage == SSAM_BF16 || storage == SSAM_F16;
 
 // Tile shape of a 3D group: the plan's, except that BF16 / F16 groups always run
 // the two-sweep shape (the only one instantiated for 2-byte storage and for the
-// FP32 state between their passes).
-int var3d_of(const Plan* P, const Plan::Group& g) { return two_byte(g.storage) ? kVarT2 : P->var3d; }
+// FP32 state between their passes: the three-sweep tile reading 2-byte planes
+// spilled registers, 248 VGPRs + 92 B of scratch, and ran the config-4 3D items
+// in 2.20 ms against 1.84), and 19-point groups take two sweeps where the plan's
+// shape has three.
+int var3d_of(const Plan* P, const Plan::Group& g) {
+    if (two_byte(g.storage)) return kVarT2;
+    return P->var3d == kVarT3 && g.kind ? kVarT2 : P->var3d;
+}
 
-// 3D launch form in effect: the two-sweep shape always runs as an ordinary grid.
-bool grid3d_on(const Plan* P, const Plan::Group& g) { return P->grid3d || kVar3D[var3d_of(P, g)].t == 2; }
+// 3D launch form in effect: the multi-sweep shapes always run as ordinary grids.
+bool grid3d_on(const Plan* P, const Plan::Group& g) { return P->grid3d || kVar3D[var3d_of(P, g)].t >= 2; }
 
 int elem_size_b(int dtype) {
     switch (dtype) {
@@ -370,7 +376,7 @@ int build_group3d(Plan* P, Plan::Group& g) {
         b.fp32_state = two_byte(g.storage) && b.steps > 1;
         const int T = kVar3D[variant].t;
         const int V = var3d_v(b.var, sizeof(A)), NY = kVar3D[b.var].ny;
-        const int OW = (kWave - 2 * (T == 2 && V == 1 ? 2 : 1)) * V;  // halo lanes: two columns per sweep
+        const int OW = (kWave - 2 * var3d_hl(variant, V)) * V;  // halo lanes: a column per sweep
         const int nt = g.kind ? 19 : 7;
         for (int t = 0; t < nt; ++t) taps.push_back(static_cast<A>(it.coeffs[t]));
         Grid3D& gr = b.g;
@@ -518,8 +524,8 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             if (rc != SSAM_OK) return rc;
         }
     }
-    // passes: one sweep each, or two (temporal variants; the last pass of an odd
-    // sweep count runs one)
+    // passes: one sweep each, two (the last pass of an odd sweep count runs one)
+    // or three (four sweeps as 2+2, the rest twos and ones)
     int maxsteps = 0;
     for (auto& g : P->groups)
         if (g.ndim == 3) maxsteps = std::max(maxsteps, g.steps);
@@ -528,10 +534,13 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             if (g.ndim != 3) continue;
             int* cnt = counters + kCounterSet * ctr++;
             const int var = var3d_of(P, g);
-            const bool t2 = kVar3D[var].t == 2;
+            const int t = kVar3D[var].t;
             const int npass = passes3d(g.steps, var);
             if (!(parts & kRun3D) || s >= npass) continue;
-            int single = t2 && (g.steps % 2 == 1) && s == npass - 1;
+            // the three-sweep kernel takes the pass's sweep count, the two-sweep
+            // one a flag for its one-sweep last pass
+            int single = t == 3 ? pass_sweeps3d(g.steps, var, s)
+                                : (t == 2 && (g.steps % 2 == 1) && s == npass - 1);
             // storage read and written by this pass (FP32 between the passes of a
             // BF16 / F16 group)
             int in_st = g.storage, out_st = g.storage;
@@ -817,8 +826,8 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream) {
         // tile shape x z-segment length x launch form (persistent / grid), then
         // the persistent grid's occupancy, queue affinity and step direction
         search({&P->var3d, &P->zseg3d, &P->grid3d},
-               {{0, 1, 2, 3, kVarPair3D, kVarT2}, {16, 32, 64}, {0, 1}}, kRun3D);
-        if (!P->grid3d && kVar3D[P->var3d].t != 2) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
+               {{0, 1, 2, 3, kVarPair3D, kVarT2, kVarT3}, {16, 32, 64}, {0, 1}}, kRun3D);
+        if (!P->grid3d && kVar3D[P->var3d].t < 2) search({&P->wps3d}, {{1, 2, 4}}, kRun3D);
         search({&P->aff3d, &P->alt3d}, {{0, 1}, {0, 1}}, kRun3D);
     }
     const int rc2 = build_all(P);
@@ -836,7 +845,7 @@ int ssam_batch_plan_tiling(ssam_batch_plan plan, int32_t* seg_rows_2d, int32_t* 
         sched[4] = plan->mfma2d;
         sched[5] = plan->wps2d;
         sched[6] = plan->wps3d;
-        sched[7] = plan->grid3d || kVar3D[plan->var3d].t == 2 ? 1 : 0;
+        sched[7] = plan->grid3d || kVar3D[plan->var3d].t >= 2 ? 1 : 0;
         sched[8] = plan->grid2d;
     }
     if (seg_rows_2d)

@@ -1,8 +1,10 @@
 // batch3d_kernel.hpp -- the batched 3D kernel template (instantiated by
-// batch3d.hip for the one-sweep tile shapes and by batch3d_t2.hip for the
-// two-sweep shape, so the two compile in parallel).
+// batch3d.hip for the one-sweep tile shapes, by batch3d_t2*.hip for the two-sweep
+// shape and by batch3d_t3*.hip for the three-sweep one, so they compile in
+// parallel).
 #pragma once
 #include "batch_common.hpp"
+#include "stencil3d_hlt.hpp"
 
 namespace ssam {
 
@@ -16,14 +18,28 @@ template <typename S> struct Waves3D<S, kVarPair3D> {
     static constexpr int value = a < b ? a : b;
 };
 
+// single: the pass runs one sweep (two-sweep shape, the last pass of an odd
+// count); for the three-sweep shape it is the pass's sweep count (1..3), all on
+// the tiles of three (the same halo lanes).
 template <typename SI, typename SO, int SHAPE, int VAR>
 __device__ __forceinline__ void run_tile3d(const SI* in, SO* out, const BatchItem3D& b,
                                            const typename Storage<SI>::acc* w, int strip, int yb, int zs, int ze,
-                                           bool single) {
+                                           int single) {
     using T = Var3DT<SI, VAR>;
-    if constexpr (T::T == 2) {
+    if constexpr (T::T == 3) {
+        constexpr int HL = (3 + T::V - 1) / T::V;
+        if (single == 3)
+            Stencil3DHLT<SI, SHAPE, T::V, T::NY, T::PF, SO, 3, HL>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb,
+                                                                             zs, ze);
+        else if (single == 2)
+            Stencil3DHLT<SI, SHAPE, T::V, T::NY, T::PF, SO, 2, HL>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb,
+                                                                             zs, ze);
+        else
+            Stencil3DHLT<SI, SHAPE, T::V, T::NY, T::PF, SO, 1, HL>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb,
+                                                                             zs, ze);
+    } else if constexpr (T::T == 2) {
         Stencil3DHL2<SI, SHAPE, T::V, T::NY, T::PF, SO>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze,
-                                                                   single);
+                                                                   single != 0);
     } else {
         (void)single;
         Stencil3DHL<SI, SHAPE, T::V, T::NY, T::PF, SO>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze);
@@ -64,10 +80,10 @@ void batch3d_kernel(const BatchItem3D* __restrict__ items, const int4* __restric
         SO* dst = static_cast<SO*>(out);
         const typename Storage<SI>::acc* w = taps + b.taps_off;
         if constexpr (VAR == kVarPair3D) {
-            if (b.var == kPairB) run_tile3d<SI, SO, SHAPE, kPairB>(src, dst, b, w, strip, yb, zs, ze, false);
-            else run_tile3d<SI, SO, SHAPE, kPairA>(src, dst, b, w, strip, yb, zs, ze, false);
+            if (b.var == kPairB) run_tile3d<SI, SO, SHAPE, kPairB>(src, dst, b, w, strip, yb, zs, ze, 0);
+            else run_tile3d<SI, SO, SHAPE, kPairA>(src, dst, b, w, strip, yb, zs, ze, 0);
         } else {
-            run_tile3d<SI, SO, SHAPE, VAR>(src, dst, b, w, strip, yb, zs, ze, single != 0);
+            run_tile3d<SI, SO, SHAPE, VAR>(src, dst, b, w, strip, yb, zs, ze, single);
         }
     };
     if constexpr (GRID) {
@@ -85,6 +101,11 @@ using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, cons
                            int);
 
 // The two-sweep kernel of one storage pair and shape (batch3d_t2*.hip).
+template <typename SI, typename SO>
+const void* batch3d_t3_fn() {
+    return reinterpret_cast<const void*>(batch3d_kernel<SI, SO, kVarT3, k3Star, true>);
+}
+
 template <typename SI, typename SO>
 const void* batch3d_t2_fn(int shape3d) {
     return shape3d ? reinterpret_cast<const void*>(batch3d_kernel<SI, SO, kVarT2, k3FaceEdge, true>)

@@ -160,12 +160,16 @@ struct Var3D { int v, ny, pf, wpe, t; };
 // reach of the best on MI355X (a wider menu only lengthened the build).
 constexpr int kVarPair3D = 4;
 constexpr int kPairA = 2, kPairB = 3;
-// kVarT2: two sweeps per pass, 2 columns x 6 rows per lane (the fastest on the
-// uneven batch; always an ordinary grid)
+// kVarT2: two sweeps per pass, 2 columns x 6 rows per lane (always an ordinary
+// grid).  kVarT3: three sweeps per pass for 7-point FP32 / FP64 items
+// (Stencil3DHLT), 2 columns x 4 rows per lane; its passes of two or one sweep
+// (the tail of a count) run the same tiles; 19-point and 2-byte items take kVarT2
+// instead.
 constexpr int kVarT2 = 5;
+constexpr int kVarT3 = 6;
 constexpr Var3D kVar3D[] = {{1, 8, 2, 4, 1}, {2, 8, 2, 2, 1}, {2, 12, 2, 2, 1}, {4, 8, 1, 2, 1},
-                            {2, 12, 2, 2, 1}, {2, 6, 1, 3, 2}};
-constexpr int kNumVar3D = 6;
+                            {2, 12, 2, 2, 1}, {2, 6, 1, 3, 2},  {2, 4, 1, 2, 3}};
+constexpr int kNumVar3D = 7;
 constexpr int kDefaultVar3D = kVarT2;
 constexpr int kDefaultZseg3D = 16;
 constexpr int kDefaultSeg2D = 64;
@@ -176,13 +180,33 @@ template <typename S, int VAR> struct Var3DT {
     // FP64 runs one column per lane (a plane row of NY+2 doubles per lane)
     static constexpr int V = sizeof(A) == 8 ? 1 : kVar3D[VAR].v;
     static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
-    static constexpr int WPE = sizeof(A) == 8 ? (NY >= 16 || kVar3D[VAR].t == 2 ? 1 : 2) : kVar3D[VAR].wpe;
+    static constexpr int WPE = sizeof(A) == 8 ? (NY >= 16 || kVar3D[VAR].t >= 2 ? 1 : 2) : kVar3D[VAR].wpe;
     static constexpr int T = kVar3D[VAR].t;
 };
 
-// Passes of a 3D item of `steps` sweeps with a variant of t sweeps per pass (the
-// last pass of an odd count runs one sweep).
-inline int passes3d(int steps, int variant) { return kVar3D[variant].t == 2 ? (steps + 1) / 2 : steps; }
+// Sweeps in each pass of a 3D item of `steps` sweeps with a variant of t sweeps
+// per pass: t = 2 -- twos, the last pass of an odd count one; t = 3 -- threes,
+// four as 2+2, the rest twos and ones (as ssam_stencil3d plans its launches).
+inline int pass_sweeps3d(int steps, int variant, int pass) {
+    const int t = kVar3D[variant].t;
+    int left = steps;
+    for (int p = 0;; ++p) {
+        const int n = t == 3 ? ((left >= 3 && left != 4) ? 3 : (left >= 2 ? 2 : 1))
+                             : (t == 2 && left >= 2 ? 2 : 1);
+        if (p == pass || left <= n) return p == pass ? n : 0;
+        left -= n;
+    }
+}
+inline int passes3d(int steps, int variant) {
+    int n = 0;
+    for (int left = steps; left > 0; left -= pass_sweeps3d(left, variant, 0)) ++n;
+    return n;
+}
+// Halo lanes per strip side of a variant (columns of every sweep of a pass).
+inline int var3d_hl(int variant, int v) {
+    const int t = kVar3D[variant].t;
+    return t == 3 ? (3 + v - 1) / v : (t == 2 && v == 1 ? 2 : 1);
+}
 
 // Columns per lane of a variant for storage of `elem_bytes` bytes.
 inline int var3d_v(int variant, int elem_bytes) { return elem_bytes == 8 ? 1 : kVar3D[variant].v; }
@@ -214,5 +238,7 @@ const void* batch3d_kernel_fn(int storage, int out_storage, int variant, int sha
 const void* batch3d_t2_kernel_fn(int storage, int out_storage, int shape3d);
 const void* batch3d_t2_kernel_fn_bf16(int storage, int out_storage, int shape3d);
 const void* batch3d_t2_kernel_fn_f16(int storage, int out_storage, int shape3d);
+// The three-sweep kernel (7-point star; FP32 / FP64) by storage (batch3d_t3.hip).
+const void* batch3d_t3_kernel_fn(int storage, int out_storage);
 
 }  // namespace ssam

@@ -1,10 +1,11 @@
-// stencil3d_hlt.hpp -- S sweeps of a radius-1 3D stencil per launch (S >= 2):
+// stencil3d_hlt.hpp -- S sweeps of a radius-1 3D stencil per launch (S >= 1):
 // the temporal blocking of Stencil3DHL2 carried to more sweeps, so a multi-sweep
 // run reads and writes the grid once per S sweeps (the reference's IT fused
 // steps, ssai-3dstencil/tmp3d7pt-v2.cu:62-208, generalised).
 //
-// A wave owns an x-strip (HL = ceil(S/V) halo lanes per side, whose columns
-// only feed their neighbours) and a y-block of NY output rows, and streams z.
+// A wave owns an x-strip (HL = ceil(S/V) halo lanes per side, or HLX if more,
+// whose columns only feed their neighbours) and a y-block of NY output rows, and
+// streams z.  (HLX lets passes of fewer sweeps share the tiles of a longer one.)
 // Sweep k (k = 1..S) works one plane ahead of sweep k+1: at tick q it forms
 // plane q+S-k over rows y0-(S-k) .. y0+NY-1+(S-k) from the three neighbouring
 // planes of sweep k-1 (sweep 0 = the input ring of NY+2S rows), and keeps it in
@@ -26,16 +27,17 @@
 
 namespace ssam {
 
-template <typename SI, int SHAPE, int V, int NY, int PF, typename SO, int S>
+template <typename SI, int SHAPE, int V, int NY, int PF, typename SO, int S, int HLX = 0>
 struct Stencil3DHLT {
     using A = typename Storage<SI>::acc;
     static_assert(std::is_same<A, typename Storage<SO>::acc>::value, "one accumulation type");
-    static_assert(PF >= 1 && S >= 2, "prefetch at least one plane; two sweeps or more");
+    static_assert(PF >= 1 && S >= 1, "prefetch at least one plane; one sweep or more");
     static constexpr int NT = ntaps3(SHAPE, 1);
-    static constexpr int HL = (S + V - 1) / V;       // halo lanes per side
+    static constexpr int HL = (S + V - 1) / V > HLX ? (S + V - 1) / V : HLX;  // halo lanes per side
     static constexpr int OW = (kWave - 2 * HL) * V;  // output columns per strip
     static constexpr int NI = NY + 2 * S;            // input rows y0-S .. y0+NY-1+S
     static constexpr int NM = NY + 2 * (S - 1);      // rows of sweep 1 (sweep k uses NY+2(S-k))
+    static constexpr int NMR = S > 1 ? S - 1 : 1;    // intermediate rings (none used when S = 1)
     static constexpr int NPI = 3 + PF;               // input planes in the ring
     static constexpr int PERIOD = clcm(NPI, 3);      // ticks until all rings realign
     static constexpr uint32_t ESI = sizeof(SI), ESO = sizeof(SO);
@@ -74,7 +76,7 @@ struct Stencil3DHLT {
     // zq0+j in slot j % 3 of mid[k-1].
     template <bool VEC, int KT>
     __device__ static __forceinline__ void tick(const Ctx& c, const A* __restrict__ w, int zq0,
-                                                PlaneI (&in)[NPI], PlaneM (&mid)[S - 1][3]) {
+                                                PlaneI (&in)[NPI], PlaneM (&mid)[NMR][3]) {
         const int q = zq0 + KT;
         load_plane<VEC>(c, q + S + PF, in[(KT + S + PF) % NPI]);
         int zero = 0;
@@ -118,7 +120,9 @@ struct Stencil3DHLT {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
                         const Tap3 tp = tap3(SHAPE, t);
-                        acc = fmaa(tw[t], at(mid[S - 2][(KT + tp.dz + 3) % 3].v, i + 1 + tp.dy, j, tp.dx), acc);
+                        const A x = S == 1 ? at(in[(KT + tp.dz + NPI) % NPI].v, i + 1 + tp.dy, j, tp.dx)
+                                           : at(mid[S > 1 ? S - 2 : 0][(KT + tp.dz + 3) % 3].v, i + 1 + tp.dy, j, tp.dx);
+                        acc = fmaa(tw[t], x, acc);
                     }
                     out[j] = (yin && c.xin[j]) ? acc : A(0);
                 }
@@ -129,7 +133,7 @@ struct Stencil3DHLT {
 
     template <bool VEC, int KT>
     __device__ static __forceinline__ void period(const Ctx& c, const A* __restrict__ w, int zq0,
-                                                  PlaneI (&in)[NPI], PlaneM (&mid)[S - 1][3]) {
+                                                  PlaneI (&in)[NPI], PlaneM (&mid)[NMR][3]) {
         if constexpr (KT < PERIOD) {
             tick<VEC, KT>(c, w, zq0, in, mid);
             period<VEC, KT + 1>(c, w, zq0, in, mid);
@@ -138,7 +142,7 @@ struct Stencil3DHLT {
 
     template <bool VEC, int KT>
     __device__ static __forceinline__ void tail(const Ctx& c, const A* __restrict__ w, int zq0, int zend,
-                                                PlaneI (&in)[NPI], PlaneM (&mid)[S - 1][3]) {
+                                                PlaneI (&in)[NPI], PlaneM (&mid)[NMR][3]) {
         if constexpr (KT < PERIOD - 1) {
             if (zq0 + KT < zend) {
                 tick<VEC, KT>(c, w, zq0, in, mid);
@@ -150,11 +154,11 @@ struct Stencil3DHLT {
     template <bool VEC>
     __device__ static __forceinline__ void stream(const Ctx& c, const A* __restrict__ w, int zs, int ze) {
         PlaneI in[NPI];
-        PlaneM mid[S - 1][3];
+        PlaneM mid[NMR][3];
         // the first ticks read intermediate planes no sweep has formed yet (their
         // results feed only planes that are never stored): start them at 0
 #pragma unroll
-        for (int k = 0; k < S - 1; ++k)
+        for (int k = 0; k < NMR; ++k)
 #pragma unroll
             for (int s = 0; s < 3; ++s)
 #pragma unroll
@@ -165,7 +169,7 @@ struct Stencil3DHLT {
         // q+S-2 .. q+S+PF-1 (slots S-2 .. S+PF-1, mod NPI)
         const int zq = zs - 2 * (S - 1);
 #pragma unroll
-        for (int k = S - 2; k < S + PF; ++k) load_plane<VEC>(c, zq + k, in[k % NPI]);
+        for (int k = S - 2; k < S + PF; ++k) load_plane<VEC>(c, zq + k, in[(k + NPI) % NPI]);
         int zq0 = zq;
 #pragma nounroll
         for (; zq0 + PERIOD <= ze; zq0 += PERIOD) period<VEC, 0>(c, w, zq0, in, mid);

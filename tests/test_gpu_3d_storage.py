@@ -123,13 +123,16 @@ def test_jacobi19_fp32_state_and_fp64_residual(storage, tol):
     assert abs(res - np.sqrt(r2)) <= 5 * tol * np.sqrt(r2), (res, np.sqrt(r2))
 
 
+@pytest.mark.parametrize("variant", ["5", "6"])
 @pytest.mark.parametrize("storage,tol", STORAGE)
-def test_batch_3d_items_in_2byte_storage(storage, tol):
-    """BF16 / FP16 3D items in a batch plan: the two-sweep tile shape, FP32 state
+def test_batch_3d_items_in_2byte_storage(storage, tol, variant, monkeypatch):
+    """BF16 / FP16 3D items in a batch plan (the plan's shape the two- or the
+    three-sweep one: 2-byte items run two sweeps per pass either way), FP32 state
     between passes in each item's work (two FP32 grids), graph replays and an
     eager run against the FP64 gold."""
+    monkeypatch.setenv("SSAM_BATCH3D_VARIANT", variant)
     specs = [(70, 33, 20, "star", 1), (64, 64, 64, "face_edge", 1), (257, 9, 11, "star", 10),
-             (40, 40, 40, "face_edge", 3), (33, 17, 9, "star", 2)]
+             (40, 40, 40, "face_edge", 3), (33, 17, 9, "star", 2), (50, 21, 30, "star", 7)]
     items, refs = [], []
     for k, (nx, ny, nz, shape, steps) in enumerate(specs):
         x = oracle.uniform(500 + k, nx * ny * nz, -1, 1, np.float64).reshape(nz, ny, nx)

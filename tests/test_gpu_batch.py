@@ -123,7 +123,7 @@ def test_batch_tune_keeps_results():
     plan = api.BatchPlan(items)
     t = plan.tune()
     assert set(t["seg_rows_2d"].values()) <= {32, 64, 128, 256}
-    assert 0 <= t["variant_3d"] <= 5 and t["zseg_3d"] in (16, 32, 64)
+    assert 0 <= t["variant_3d"] <= 6 and t["zseg_3d"] in (16, 32, 64)
     assert t["affinity_2d"] in (0, 1) and t["affinity_3d"] in (0, 1)
     assert t["split_2d"] in (0, 1) and t["mfma_2d"] in (0, 1, 2, 3)
     assert t["wps_2d"] in (0, 1, 2, 4) and t["wps_3d"] in (1, 2, 4) and t["grid_3d"] in (0, 1)
@@ -151,15 +151,16 @@ _SIZES3 = [(70, 33, 20), (1, 1, 1), (2, 3, 2), (3, 1, 5), (63, 5, 4), (62, 2, 3)
            (124, 8, 9), (200, 17, 33), (64, 64, 12)]
 
 
-@pytest.mark.parametrize("variant,affinity", [(v, v % 2) for v in range(6)])
+@pytest.mark.parametrize("variant,affinity", [(v, v % 2) for v in range(7)])
 @pytest.mark.parametrize("storage", [torch.float32, torch.float64])
 def test_batch3d_tile_shapes(variant, affinity, storage, monkeypatch):
     """Every 3D tile shape of the batched launch (columns / rows per lane, prefetch
-    depth; one sweep per pass, the per-grid pair, and two sweeps per pass, whose
-    odd step count ends in a one-sweep pass) on tiny, odd-width and multi-strip
-    grids, short z segments, two step counts (two groups), persistent grids of 1,
-    2 and all waves per SIMD and ordinary one-tile-per-wave grids, graph replay;
-    against the oracle."""
+    depth; one sweep per pass, the per-grid pair, two sweeps per pass, whose odd
+    step count ends in a one-sweep pass, and three, whose counts split into
+    passes of three, two and one) on tiny, odd-width and multi-strip grids, short
+    z segments, six step counts (groups), persistent grids of 1, 2 and all waves
+    per SIMD and ordinary one-tile-per-wave grids, graph replay; against the
+    oracle."""
     monkeypatch.setenv("SSAM_BATCH3D_WPS", str(variant % 3))
     monkeypatch.setenv("SSAM_BATCH3D_GRID", str((variant // 2) % 2))
     monkeypatch.setenv("SSAM_BATCH3D_VARIANT", str(variant))
@@ -169,7 +170,7 @@ def test_batch3d_tile_shapes(variant, affinity, storage, monkeypatch):
     items, refs = [], []
     for k, (nx, ny, nz) in enumerate(_SIZES3):
         shape = "face_edge" if k % 3 == 1 else "star"
-        steps = 3 if k % 2 == 0 else 2
+        steps = (3, 2, 7, 1, 5, 4)[k % 6]
         x = oracle.uniform(500 + k, nx * ny * nz, 0, 1, np.float64).reshape(nz, ny, nx)
         src = torch.from_numpy(x).to(storage).cuda()
         dst = torch.full_like(src, float("nan"))
