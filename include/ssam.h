@@ -230,9 +230,9 @@ int ssam_jacobi19_noalloc(int storage, const void* src, void* dst, int64_t nx, i
  * storage type whose shape is a 5-point star or a 3x3 / 5x5 / 11x11 box run as
  * one launch per stencil kind (or one for all kinds); 3D items (7-point star or
  * 19-point face+edge, any storage, under 2 GiB) run one launch per pass of one,
- * two or three sweeps for each (storage, steps, shape) -- BF16 / F16 items two
- * sweeps per pass with their state FP32 between passes; any other item gets its
- * own launch.
+ * two or three sweeps for each (storage, steps, shape) -- BF16 / F16 items two or
+ * three sweeps per pass with their state FP32 between passes; any other item gets
+ * its own launch.
  * Each launch is an ordinary grid with one tile per wave over a table sorted by
  * decreasing tile cost (default), or a persistent grid taking tiles from 8 device
  * work queues, one per XCD (neighbouring tiles of a grid share that XCD's L2),
@@ -277,7 +277,7 @@ int ssam_batch_plan_tune(ssam_batch_plan plan, void* stream);
 /* The plan's tiling: 2D tile height per shape (4 entries: 5-point star, 3x3,
  * 5x5, 11x11 box), 3D tile-shape index (0..3 one sweep per pass; 4: per grid,
  * the better of shapes 2 and 3 by width; 5: two sweeps per pass; 6: three sweeps
- * per pass for 7-point FP32 / FP64 items, two for the others) and z-segment length, and 9
+ * per pass for 7-point items, two for 19-point ones) and z-segment length, and 9
  * scheduling
  * entries: queue affinity of the 2D and of the 3D launches (1: waves start at
  * their own XCD's queue, 0: all waves work through the tiles in table order),

@@ -63,6 +63,8 @@ SOURCES = [
     "batch3d_t2_bf16.hip",
     "batch3d_t2_f16.hip",
     "batch3d_t3.hip",
+    "batch3d_t3_bf16.hip",
+    "batch3d_t3_f16.hip",
     "microbench.hip",
 ]
 

@@ -91,15 +91,13 @@ using Plan = ssam_batch_plan_s;
 
 bool two_byte(int storage) { return storage == SSAM_BF16 || storage == SSAM_F16; }
 
-// Tile shape of a 3D group: the plan's, except that BF16 / F16 groups always run
-// the two-sweep shape (the only one instantiated for 2-byte storage and for the
-// FP32 state between their passes: the three-sweep tile reading 2-byte planes
-// spilled registers, 248 VGPRs + 92 B of scratch, and ran the config-4 3D items
-// in 2.20 ms against 1.84), and 19-point groups take two sweeps where the plan's
-// shape has three.
+// Tile shape of a 3D group: the plan's, except that BF16 / F16 groups run a
+// multi-sweep shape (the only ones instantiated for 2-byte storage and for the
+// FP32 state between their passes: three sweeps when the plan's is, else two),
+// and 19-point groups take two sweeps where the plan's shape has three.
 int var3d_of(const Plan* P, const Plan::Group& g) {
-    if (two_byte(g.storage)) return kVarT2;
-    return P->var3d == kVarT3 && g.kind ? kVarT2 : P->var3d;
+    if (P->var3d == kVarT3) return g.kind ? kVarT2 : kVarT3;
+    return two_byte(g.storage) ? kVarT2 : P->var3d;
 }
 
 // 3D launch form in effect: the multi-sweep shapes always run as ordinary grids.
@@ -537,10 +535,14 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             const int t = kVar3D[var].t;
             const int npass = passes3d(g.steps, var);
             if (!(parts & kRun3D) || s >= npass) continue;
-            // the three-sweep kernel takes the pass's sweep count, the two-sweep
-            // one a flag for its one-sweep last pass
-            int single = t == 3 ? pass_sweeps3d(g.steps, var, s)
-                                : (t == 2 && (g.steps % 2 == 1) && s == npass - 1);
+            // the two-sweep kernel takes a flag for its one-sweep last pass; a
+            // three-sweep shape launches the kernel of the pass's sweep count
+            int single = t == 2 && (g.steps % 2 == 1) && s == npass - 1;
+            int kvar = var;
+            if (t == 3) {
+                const int nsw = pass_sweeps3d(g.steps, var, s);
+                kvar = nsw == 3 ? kVarT3 : (nsw == 2 ? kVarT3S2 : kVarT3S1);
+            }
             // storage read and written by this pass (FP32 between the passes of a
             // BF16 / F16 group)
             int in_st = g.storage, out_st = g.storage;
@@ -556,7 +558,7 @@ int enqueue(Plan* P, hipStream_t st, unsigned parts = kRunAll) {
             int step = s;
             const void* taps = g.d_taps;
             void* args[] = {&items, &tiles, &qs, &cnt, &taps, &step, &single};
-            const void* fn = batch3d_kernel_fn(in_st, out_st, var, g.kind, grid3d_on(P, g));
+            const void* fn = batch3d_kernel_fn(in_st, out_st, kvar, g.kind, grid3d_on(P, g));
             if (!fn) return SSAM_E_UNSUPPORTED;
             if (hipLaunchKernel(fn, dim3(blocks), dim3(kWave * kBatchWavesPerBlock), args, 0, st) !=
                     hipSuccess ||

@@ -27,7 +27,7 @@ template <typename S> const void* batch3d_fn_t(int variant, int shape3d, bool gr
 }
 
 const void* batch3d_kernel_fn(int storage, int out_storage, int variant, int shape3d, bool grid) {
-    if (kVar3D[variant].t == 3) return shape3d ? nullptr : batch3d_t3_kernel_fn(storage, out_storage);
+    if (kVar3D[variant].t == 3) return shape3d ? nullptr : batch3d_t3_kernel_fn(storage, out_storage, variant);
     if (kVar3D[variant].t == 2) return batch3d_t2_kernel_fn(storage, out_storage, shape3d);
     // one-sweep shapes: F32 and F64 (BF16 / F16 items always run the two-sweep shape)
     if (storage != out_storage) return nullptr;

@@ -19,24 +19,18 @@ template <typename S> struct Waves3D<S, kVarPair3D> {
 };
 
 // single: the pass runs one sweep (two-sweep shape, the last pass of an odd
-// count); for the three-sweep shape it is the pass's sweep count (1..3), all on
-// the tiles of three (the same halo lanes).
+// count).  A three-sweep shape's kernel runs T::NSW sweeps on the tiles of three
+// (the same halo lanes).
 template <typename SI, typename SO, int SHAPE, int VAR>
 __device__ __forceinline__ void run_tile3d(const SI* in, SO* out, const BatchItem3D& b,
                                            const typename Storage<SI>::acc* w, int strip, int yb, int zs, int ze,
                                            int single) {
     using T = Var3DT<SI, VAR>;
     if constexpr (T::T == 3) {
+        (void)single;
         constexpr int HL = (3 + T::V - 1) / T::V;
-        if (single == 3)
-            Stencil3DHLT<SI, SHAPE, T::V, T::NY, T::PF, SO, 3, HL>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb,
-                                                                             zs, ze);
-        else if (single == 2)
-            Stencil3DHLT<SI, SHAPE, T::V, T::NY, T::PF, SO, 2, HL>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb,
-                                                                             zs, ze);
-        else
-            Stencil3DHLT<SI, SHAPE, T::V, T::NY, T::PF, SO, 1, HL>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb,
-                                                                             zs, ze);
+        Stencil3DHLT<SI, SHAPE, T::V, T::NY, T::PF, SO, T::NSW, HL>::run_tile(in, out, b.g, b.vec != 0, w, strip,
+                                                                              yb, zs, ze);
     } else if constexpr (T::T == 2) {
         Stencil3DHL2<SI, SHAPE, T::V, T::NY, T::PF, SO>::run_tile(in, out, b.g, b.vec != 0, w, strip, yb, zs, ze,
                                                                    single != 0);
@@ -102,7 +96,9 @@ using Batch3DFn = void (*)(const BatchItem3D*, const int4*, QueueSet, int*, cons
 
 // The two-sweep kernel of one storage pair and shape (batch3d_t2*.hip).
 template <typename SI, typename SO>
-const void* batch3d_t3_fn() {
+const void* batch3d_t3_fn(int variant) {
+    if (variant == kVarT3S2) return reinterpret_cast<const void*>(batch3d_kernel<SI, SO, kVarT3S2, k3Star, true>);
+    if (variant == kVarT3S1) return reinterpret_cast<const void*>(batch3d_kernel<SI, SO, kVarT3S1, k3Star, true>);
     return reinterpret_cast<const void*>(batch3d_kernel<SI, SO, kVarT3, k3Star, true>);
 }
 

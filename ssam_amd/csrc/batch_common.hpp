@@ -153,7 +153,10 @@ template <typename S> constexpr bool kHalf2D = std::is_same<S, bf16_t>::value ||
 // wpe: waves per SIMD the register allocation must leave room for (bounds the
 // compiler's load hoisting to what each shape needs).
 // t: sweeps per pass (2: temporal blocking, Stencil3DHL2).
-struct Var3D { int v, ny, pf, wpe, t; };
+// s: the sweeps a kernel of a three-sweep shape runs (0: t); the shape's passes
+// of two and one sweep (the tail of a count) launch kernels of their own on the
+// same tiles (one kernel running all three tile bodies spilled registers).
+struct Var3D { int v, ny, pf, wpe, t; int s = 0; };
 // Variant index kVarPair3D: per item, the better of shapes kPairA (narrow
 // grids) and kPairB (wider ones), in one kernel (pair_choice below); its table
 // entry repeats kPairA's shape.  The shapes kept are those that measured within
@@ -161,15 +164,16 @@ struct Var3D { int v, ny, pf, wpe, t; };
 constexpr int kVarPair3D = 4;
 constexpr int kPairA = 2, kPairB = 3;
 // kVarT2: two sweeps per pass, 2 columns x 6 rows per lane (always an ordinary
-// grid).  kVarT3: three sweeps per pass for 7-point FP32 / FP64 items
-// (Stencil3DHLT), 2 columns x 4 rows per lane; its passes of two or one sweep
-// (the tail of a count) run the same tiles; 19-point and 2-byte items take kVarT2
-// instead.
+// grid).  kVarT3: three sweeps per pass for 7-point items (Stencil3DHLT), 2
+// columns x 4 rows per lane; its passes of two or one sweep (the tail of a
+// count) run the same tiles; 19-point items take kVarT2 instead.
 constexpr int kVarT2 = 5;
 constexpr int kVarT3 = 6;
+constexpr int kVarT3S2 = 7, kVarT3S1 = 8;  // kVarT3's passes of two / one sweep (launch only)
 constexpr Var3D kVar3D[] = {{1, 8, 2, 4, 1}, {2, 8, 2, 2, 1}, {2, 12, 2, 2, 1}, {4, 8, 1, 2, 1},
-                            {2, 12, 2, 2, 1}, {2, 6, 1, 3, 2},  {2, 4, 1, 2, 3}};
-constexpr int kNumVar3D = 7;
+                            {2, 12, 2, 2, 1}, {2, 6, 1, 3, 2},  {2, 4, 1, 2, 3},  {2, 4, 1, 2, 3, 2},
+                            {2, 4, 1, 2, 3, 1}};
+constexpr int kNumVar3D = 7;  // shapes a plan may take (the tuner's and SSAM_BATCH3D_VARIANT's range)
 constexpr int kDefaultVar3D = kVarT2;
 constexpr int kDefaultZseg3D = 16;
 constexpr int kDefaultSeg2D = 64;
@@ -182,6 +186,7 @@ template <typename S, int VAR> struct Var3DT {
     static constexpr int NY = kVar3D[VAR].ny, PF = kVar3D[VAR].pf;
     static constexpr int WPE = sizeof(A) == 8 ? (NY >= 16 || kVar3D[VAR].t >= 2 ? 1 : 2) : kVar3D[VAR].wpe;
     static constexpr int T = kVar3D[VAR].t;
+    static constexpr int NSW = kVar3D[VAR].s ? kVar3D[VAR].s : T;  // sweeps this kernel runs
 };
 
 // Sweeps in each pass of a 3D item of `steps` sweeps with a variant of t sweeps
@@ -238,7 +243,10 @@ const void* batch3d_kernel_fn(int storage, int out_storage, int variant, int sha
 const void* batch3d_t2_kernel_fn(int storage, int out_storage, int shape3d);
 const void* batch3d_t2_kernel_fn_bf16(int storage, int out_storage, int shape3d);
 const void* batch3d_t2_kernel_fn_f16(int storage, int out_storage, int shape3d);
-// The three-sweep kernel (7-point star; FP32 / FP64) by storage (batch3d_t3.hip).
-const void* batch3d_t3_kernel_fn(int storage, int out_storage);
+// The kernel of a three-sweep shape's pass (variant kVarT3, kVarT3S2 or kVarT3S1;
+// 7-point star) by storage pair (batch3d_t3*.hip).
+const void* batch3d_t3_kernel_fn(int storage, int out_storage, int variant);
+const void* batch3d_t3_kernel_fn_bf16(int storage, int out_storage, int variant);
+const void* batch3d_t3_kernel_fn_f16(int storage, int out_storage, int variant);
 
 }  // namespace ssam

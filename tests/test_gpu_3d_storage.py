@@ -126,10 +126,9 @@ def test_jacobi19_fp32_state_and_fp64_residual(storage, tol):
 @pytest.mark.parametrize("variant", ["5", "6"])
 @pytest.mark.parametrize("storage,tol", STORAGE)
 def test_batch_3d_items_in_2byte_storage(storage, tol, variant, monkeypatch):
-    """BF16 / FP16 3D items in a batch plan (the plan's shape the two- or the
-    three-sweep one: 2-byte items run two sweeps per pass either way), FP32 state
-    between passes in each item's work (two FP32 grids), graph replays and an
-    eager run against the FP64 gold."""
+    """BF16 / FP16 3D items in a batch plan: the two- and three-sweep tile shapes,
+    FP32 state between passes in each item's work (two FP32 grids), graph replays
+    and an eager run against the FP64 gold."""
     monkeypatch.setenv("SSAM_BATCH3D_VARIANT", variant)
     specs = [(70, 33, 20, "star", 1), (64, 64, 64, "face_edge", 1), (257, 9, 11, "star", 10),
              (40, 40, 40, "face_edge", 3), (33, 17, 9, "star", 2), (50, 21, 30, "star", 7)]
