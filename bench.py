@@ -68,6 +68,10 @@ def parse():
     ap.add_argument("--ingest-sq", nargs="+", metavar="ARG",
                     help="NAME COMMAND DIR [DIR ...]: summarise rocprofv3 --pmc runs of SQ / GRBM "
                          "counters (one directory per pass) into profiles/NAME.md and exit")
+    ap.add_argument("--ingest-3d-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU: turn two rocprofv3 --pmc runs of --side config2 (FETCH_SIZE, then "
+                         "WRITE_SIZE) into HBM bytes per sweep of each FP32 / FP64 3D kernel under "
+                         "profiles/ and exit")
     ap.add_argument("--ingest-stats", nargs=3, metavar=("STATS_DIR", "NAME", "COMMAND"),
                     help="no GPU: summarise a rocprofv3 --kernel-trace --stats run (its "
                          "kernel_stats.csv) into profiles/NAME.md, quoting COMMAND, and exit")
@@ -90,6 +94,51 @@ def ingest_stats(stats_dir: str, name: str, command: str) -> None:
                    f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
                    f"{100 * float(r['TotalDurationNs']) / total:.1f}% |")
     dst = pathlib.Path(__file__).resolve().parent / "profiles" / f"{name}.md"
+    dst.write_text("\n".join(out) + "\n")
+    print(dst)
+
+
+def ingest_3d_traffic(fetch_dir: str, write_dir: str) -> None:
+    """HBM bytes per dispatch of the FP32 / FP64 3D kernels of a --side config2 run
+    (512^3), from FETCH_SIZE and WRITE_SIZE (KiB per dispatch, scaled x2 and x1:
+    the factors that make them read the bytes of the stream copy), as multiples of
+    the grid's bytes, and per sweep (the launch's sweeps: ssam3d_t3_kernel three,
+    ssam3d_t2_kernel two, others one)."""
+    import collections
+    import csv
+
+    def load(d, counter):
+        per = collections.defaultdict(list)
+        for r in csv.DictReader(open(next(pathlib.Path(d).rglob("*counter_collection.csv")))):
+            if "ssam3d" in r["Kernel_Name"] and r["Counter_Name"] == counter:
+                per[r["Kernel_Name"].split("(")[0].replace("void ssam::", "")].append(float(r["Counter_Value"]))
+        return per
+
+    fe, wr = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    rows = []
+    for k, v in fe.items():
+        targ = k.split("<", 1)[1]
+        if not targ.startswith(("float,", "double,")) or not targ.rstrip(">").endswith(("float", "double")):
+            continue
+        grid = 512 ** 3 * (8 if targ.startswith("double") else 4)
+        sweeps = 3 if "t3_kernel" in k else (2 if "t2_kernel" in k else 1)
+        f = sum(v) / len(v) * 1024 * 2 / grid
+        w = sum(wr[k]) / len(wr[k]) * 1024 / grid if wr.get(k) else float("nan")
+        rows.append(f"| `{k}` | {sweeps} | {len(v)} | {f:.2f} | {w:.2f} | {(f + w) / sweeps:.2f} |")
+    cmd = "\n".join(f"timeout -s KILL 300 rocprofv3 --pmc {c} --output-format csv -d {d} -o run -- "
+                     "python3 bench.py --side config2" for c, d in (("FETCH_SIZE", "F"), ("WRITE_SIZE", "W")))
+    out = ["# stencil3d_512_multi_sweep_traffic", "",
+           "HBM traffic per dispatch of the FP32 / FP64 3D kernels on 512³ (config 2 side run): one sweep per",
+           "launch (`ssam3d_kernel`, `ssam3d_zp_kernel`), two (`ssam3d_t2_kernel`) and three (`ssam3d_t3_kernel`,",
+           "7-point star).", "", "Commands (repository root, on one MI355X):", "", "```", cmd,
+           "python3 bench.py --ingest-3d-traffic F W", "```", "",
+           "FETCH_SIZE is scaled ×2 and WRITE_SIZE ×1, the factors that make each counter read the bytes of the",
+           "1 GiB stream copy (the suite's PMC calibration). Grid: 512 MiB FP32, 1 GiB FP64. Averages over the",
+           "dispatches of each kernel; template arguments: storage, shape (0 star, 1 face+edge), radius or",
+           "columns per lane, ..., dst storage.", "",
+           "| kernel | sweeps / launch | dispatches | FETCH (× grid) | WRITE (× grid) | HBM bytes per sweep (× grid) |",
+           "|---|---|---|---|---|---|", *rows]
+    dst = ROOT / "profiles" / "stencil3d_512_multi_sweep_traffic.md"
     dst.write_text("\n".join(out) + "\n")
     print(dst)
 
@@ -343,6 +392,9 @@ def main():
         return
     if a.ingest_sq:
         ingest_sq(a.ingest_sq[0], a.ingest_sq[1], a.ingest_sq[2:])
+        return
+    if a.ingest_3d_traffic:
+        ingest_3d_traffic(*a.ingest_3d_traffic)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SSAM stencil path runs on a single MI355X here "
