@@ -32,11 +32,14 @@ inline int kind2d(int shape, int r) {
     return k2None;
 }
 
-// BF16/FP16 tiles take 4 columns per lane (248-column strips, as FP32): the
+// BF16/FP16 box tiles take 4 columns per lane (248-column strips, as FP32): the
 // batch's grids are narrow enough that 496-column strips leave lanes idle
-// (measured 6% faster on the whole uneven batch, 30% on its 5x5 boxes).
+// (measured 6% faster on the whole uneven batch, 30% on its 5x5 boxes).  The
+// 5-point star keeps 8 (one 16-byte row access per lane): its BF16 batch items
+// ran 4% faster that way once edge tiles stopped waiting at every row.
 template <typename S, int R, int SHAPE> struct Inst2D {
-    static constexpr int V = sizeof(S) == 2 ? Lanes2D<S>::VN : Lanes2D<S>::V;
+    static constexpr int V =
+        sizeof(S) == 2 && !(SHAPE == kStar && R == 1) ? Lanes2D<S>::VN : Lanes2D<S>::V;
     using UR = Unroll2D<R, SHAPE, V, 1>;
     using K = Stencil2D<S, R, SHAPE, V, 1, UR::PF, UR::RING, UR::U>;
     using G = Geom2D<R, V, 1>;
