@@ -242,6 +242,19 @@ def test_temporal_interior_tiles_exact(r, dt):
         assert np.array_equal(got, want), f"r={r} {(h, w)}: max diff {np.max(np.abs(got - want))}"
 
 
+@pytest.mark.parametrize("storage,tol", [(torch.bfloat16, 1e-2), (torch.float16, 1e-3)])
+@pytest.mark.parametrize("r", [1, 2])
+def test_temporal_interior_tiles_2byte(storage, tol, r):
+    # Eight fused steps in 2-byte storage on grids tall and wide enough for
+    # interior tiles (segments of 64 / 128 rows with rows above and below)
+    f = wl.normalised_taps("star", r, 78)
+    for (h, w) in [(700, 600), (420, 1100)]:
+        x = oracle.uniform(9 + r, h * w, -1, 1, np.float64).reshape(h, w)
+        got = run2d(x, f, L.STAR, steps=8, storage=storage)
+        want = oracle.conv2d_iter(x, f, 8)
+        assert_close(got, want, scale_of(x, f, 8), tol, f"{storage} r={r} T=8 {(h, w)}")
+
+
 def test_box_multistep_chains_single_sweeps():
     f = wl.normalised_taps("box", 1, 5)
     x = oracle.uniform(9, 70 * 90, -1, 1, np.float64).reshape(70, 90)
