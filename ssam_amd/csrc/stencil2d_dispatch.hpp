@@ -63,7 +63,7 @@ template <typename S> constexpr int storage_of() {
 }
 
 // Per-storage-type geometry: columns per lane for a single sweep (V: one 16-byte
-// access; VN: the narrower strip of 2-byte stars of radius >= 4, boxes of radius
+// access; VN: the narrower strip of 2-byte stars of radius >= 3, boxes of radius
 // >= 3 and batch tiles), and for fused time steps of radius 1 / radius 2 (VT1 /
 // VT2), and the most steps fused per launch at radius 2.  Fused steps: fewer
 // columns keep the T-stage register rings in VGPRs with room for more waves per
@@ -344,14 +344,15 @@ int dispatch2d(const Req2D& rq, hipStream_t st) {
             (rq.form == kFormMfma || (rq.form == kFormAuto && rq.radius >= 2)))
             return launch_mfma<S>(rq, st);
         if (rq.shape == kStar) {
-            // BF16/FP16 stars of radius >= 4: 4 columns per lane (8-byte rows per
+            // BF16/FP16 stars of radius >= 3: 4 columns per lane (8-byte rows per
             // lane) keep the ring in the loop-period form with room for more
-            // waves; measured 13-36% faster than 8 columns at radius 5 and 6
+            // waves; measured 13-36% faster than 8 columns at radius 5 and 6, 8%
+            // at radius 3 (8192^2: 0.051 vs 0.055 ms)
             if constexpr (sizeof(S) == 2) {
                 switch (rq.radius) {
                     case 1: return launch2d_instance<S, 1, kStar, V, 1>(rq, st);
                     case 2: return launch2d_instance<S, 2, kStar, V, 1>(rq, st);
-                    case 3: return launch2d_instance<S, 3, kStar, V, 1>(rq, st);
+                    case 3: return launch2d_instance<S, 3, kStar, VN, 1>(rq, st);
                     case 4: return launch2d_instance<S, 4, kStar, VN, 1>(rq, st);
                     case 5: return launch2d_instance<S, 5, kStar, VN, 1>(rq, st);
                     case 6: return launch2d_instance<S, 6, kStar, VN, 1>(rq, st);
